@@ -1,0 +1,349 @@
+/*
+ * vlb_bake.h -- C ABI of the MI355X-native probe baker.
+ *
+ * This is the drop-in boundary for vulkan-light-bakery's probe-baking hot
+ * path: a triangle scene and a probe grid go in, a per-probe spherical-
+ * harmonics (SH) coefficient buffer comes out.  The reference has no FFI;
+ * its seam is the C++ class vlb::LightBaker (src/baker/light_baker.hpp:16-54)
+ * driving Vulkan RT shaders.  Each entry point below names the reference
+ * function or shader it replaces.  The C++ facade in vlb/light_baker.hpp
+ * keeps LightBaker's ctor/bake/serialize calls on top of this ABI.
+ *
+ * Conventions
+ *  - Every function returns vlb_status (0 == VLB_OK); nothing throws across
+ *    the ABI.  vlb_last_error(ctx) returns a message for the last failure on
+ *    that context (thread-local to the context, valid until the next call).
+ *  - Host pointers unless the name says "device" / "_d".  The caller owns all
+ *    buffers it passes in.
+ *  - Coefficient layout everywhere: out[probe][coef][rgb], float32, with
+ *    coef index l*(l+1)+m as in shaders/sh.comp:35-41.
+ *  - SH frame: the real SH basis of shaders/sh_common.h:26-152 is evaluated
+ *    at the world direction swizzled to (x, z, y), i.e. SH "z" is world up
+ *    (+y).  This is what sh.comp does: env_map.rgen:21 traces toVector(..).xzy
+ *    while sh.comp:30 evaluates SH at the un-swizzled toVector(..).
+ *  - A context is thread-compatible: calls on one context are serialised by
+ *    the caller (the serving entry points are the exception, they are
+ *    internally synchronised).
+ */
+#ifndef VLB_BAKE_H
+#define VLB_BAKE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VLB_ABI_VERSION 1
+
+/* ------------------------------------------------------------------ status */
+typedef int32_t vlb_status;
+enum {
+    VLB_OK              = 0,
+    VLB_ERR_INVALID     = 1,  /* bad argument / size                          */
+    VLB_ERR_DEVICE      = 2,  /* HIP runtime error                            */
+    VLB_ERR_OOM         = 3,  /* device allocation failed                     */
+    VLB_ERR_BUDGET      = 4,  /* request exceeds the context's HBM budget     */
+    VLB_ERR_NOT_FOUND   = 5,  /* unknown scene / ticket / plan                */
+    VLB_ERR_BUSY        = 6,  /* ticket not finished yet (poll)               */
+    VLB_ERR_UNSUPPORTED = 7,  /* option not supported by this build           */
+    VLB_ERR_TIMEOUT     = 8,  /* wait deadline expired                        */
+    VLB_ERR_CANCELLED   = 9,  /* request cancelled before completion          */
+    VLB_ERR_INJECTED    = 10, /* failure injected through vlb_debug_inject    */
+    VLB_ERR_NO_DEVICE   = 11  /* no HIP device visible                        */
+};
+
+/* ----------------------------------------------------------------- options */
+enum { VLB_PREC_FP32 = 0, VLB_PREC_BF16 = 1, VLB_PREC_FP16 = 2 };
+
+/* Shading of a primary hit.
+ *  LAMBERT   : albedo/pi * sum_lights E * max(N.L,0) * visibility (linear HDR),
+ *              N flipped to face the ray, procedural sky on miss.
+ *  REFERENCE : shaders/env_map.rchit:51-102 / main.rchit:95-116 -- point light,
+ *              Phong diffuse + specular, optional sRGB encode and UNORM8
+ *              quantisation of the stored radiance (the RGBA8 storage image
+ *              of env_map_generator.cpp:39). */
+enum { VLB_SHADE_LAMBERT = 0, VLB_SHADE_REFERENCE = 1 };
+
+/* Ray direction set per probe.
+ *  FIBONACCI : R directions on the Fibonacci sphere, weight 4*pi/R.
+ *  EQUIRECT  : W x H pixel centres of shaders/env_map.rgen:21 (toVector(
+ *              x2phi, y2theta).xzy), weight (2pi/W)(pi/H) sin(theta) of
+ *              shaders/sh.comp:32-33. */
+enum { VLB_DIRS_FIBONACCI = 0, VLB_DIRS_EQUIRECT = 1 };
+
+/* SH constants: LITERAL = the 6-significant-figure values of sh_common.h,
+ * EXACT = closed-form normalisation constants. */
+enum { VLB_SH_LITERAL = 0, VLB_SH_EXACT = 1 };
+
+/* Probe ordering of vlb_probe_grid. */
+enum { VLB_ORDER_LINEAR = 0, VLB_ORDER_REFERENCE_BFS = 1 };
+
+enum { VLB_LIGHT_DIRECTIONAL = 0, VLB_LIGHT_POINT = 1 };
+
+/* Reference-shading flags. */
+enum { VLB_REF_SRGB = 1u, VLB_REF_UNORM8 = 2u };
+
+/* Equirect projection variants (vlb_project_equirect). */
+enum {
+    VLB_EQUIRECT_SH_COMP     = 0, /* shaders/sh.comp:23-42        */
+    VLB_EQUIRECT_SKYBOX_COMP = 1  /* shaders/skybox_sh.comp:23-42 */
+};
+
+#define VLB_MAX_LIGHTS 8
+#define VLB_MAX_K      16
+
+/* ------------------------------------------------------------------ types */
+typedef struct vlb_ctx  vlb_ctx;
+typedef struct vlb_plan vlb_plan;
+
+typedef struct vlb_light {
+    int32_t type;    /* VLB_LIGHT_*                                           */
+    float   vec[3];  /* DIRECTIONAL: direction the light travels (unit);      */
+                     /* POINT: position                                       */
+    float   rgb[3];  /* DIRECTIONAL: irradiance; POINT: intensity (/r^2)      */
+} vlb_light;
+
+/* Procedural sky, replacing main.rmiss:37-41 whose skybox set is unbound in
+ * the baker (env_map_generator.cpp:218).  radiance(d) =
+ *   lerp(horizon, zenith, max(d.y, 0)) + sun_rgb * exp(sun_sharpness*(d.s-1)) */
+typedef struct vlb_sky {
+    float zenith_rgb[3];
+    float horizon_rgb[3];
+    float sun_dir[3];      /* unit vector towards the sun */
+    float sun_rgb[3];
+    float sun_sharpness;
+} vlb_sky;
+
+/* Parameters of shaders/main.rchit:37-47 (defaults of env_map.rchit:25,75,
+ * 81,92,95-96: light (1,10,1), bias 0.005, ambient 0, Cd 0.5, Cs 0.5, 16). */
+typedef struct vlb_ref_shading {
+    float    light_position[3];
+    float    shadow_bias;
+    float    ambient;
+    float    c_diffuse;
+    float    c_specular;
+    float    glossiness;
+    uint32_t flags;        /* VLB_REF_SRGB | VLB_REF_UNORM8 */
+} vlb_ref_shading;
+
+typedef struct vlb_bake_params {
+    /* probes (host array, num_probes * 3 floats) */
+    uint32_t         num_probes;
+    const float*     probe_xyz;
+    /* ray set */
+    int32_t          dir_mode;        /* VLB_DIRS_*                         */
+    uint32_t         rays_per_probe;  /* FIBONACCI: R (EQUIRECT: ignored)   */
+    uint32_t         equirect_w;      /* EQUIRECT only                      */
+    uint32_t         equirect_h;
+    float            t_min;           /* primary interval, env_map.rgen:22-23 */
+    float            t_max;
+    /* projection */
+    uint32_t         sh_order;        /* 2 (9 coefs) or 3 (16 coefs)        */
+    int32_t          sh_constants;    /* VLB_SH_*                           */
+    int32_t          precision;       /* VLB_PREC_*                         */
+    /* shading */
+    int32_t          shade_mode;      /* VLB_SHADE_*                        */
+    uint32_t         num_lights;      /* LAMBERT only, <= VLB_MAX_LIGHTS    */
+    const vlb_light* lights;
+    float            shadow_bias;     /* LAMBERT only                       */
+    vlb_sky          sky;
+    vlb_ref_shading  ref;
+    /* tuning (0 = automatic) */
+    uint32_t         rays_per_slice;
+    uint32_t         reserved[3];
+} vlb_bake_params;
+
+typedef struct vlb_scene_info {
+    uint32_t num_tris;
+    uint32_t num_nodes;
+    uint32_t depth;            /* inner-node levels, root = 1             */
+    uint32_t reserved;
+    uint64_t device_bytes;
+    double   build_ms;         /* device time of the BVH build            */
+    float    bounds_min[3];
+    float    bounds_max[3];
+} vlb_scene_info;
+
+typedef struct vlb_bake_stats {
+    uint64_t primary_rays;     /* probes * rays (padding excluded)        */
+    uint64_t shadow_rays;      /* shadow rays actually traced             */
+    uint64_t node_visits;      /* inner nodes fetched (stats pass only)   */
+    uint64_t tri_tests;        /* triangles tested (stats pass only)      */
+    double   trace_ms;         /* trace+shade+project kernel, last run    */
+    double   merge_ms;         /* FP64 merge kernel, last run             */
+    double   total_ms;         /* whole execute on the device, last run   */
+    uint32_t tiles;
+    uint32_t grid;
+    uint32_t rays_per_slice;
+    uint32_t slices;
+} vlb_bake_stats;
+
+/* ------------------------------------------------------------ context API */
+int32_t     vlb_abi_version(void);
+/* Replaces Application::Application (src/application.cpp:540-556): device
+ * selection, queues and pools become one HIP device + one stream.
+ * hbm_budget_bytes == 0 means "all free memory". */
+vlb_status  vlb_ctx_create(int32_t device, uint64_t hbm_budget_bytes, vlb_ctx** out);
+vlb_status  vlb_ctx_destroy(vlb_ctx* ctx);
+const char* vlb_last_error(const vlb_ctx* ctx);
+/* Number of compute units and device name (for reports). */
+vlb_status  vlb_device_info(vlb_ctx* ctx, int32_t* num_cus, char* name, size_t name_len);
+
+/* ------------------------------------------------------------- scene API */
+/* Replaces SceneManager::pushScene + Scene_t::buildAccelerationStructures
+ * (src/scene_manager.cpp:1013-1034, 339-443): world-space triangles are
+ * uploaded and a BVH is built on the GPU (LBVH, Karras 2012).
+ * xyz: nverts*3, idx: ntris*3, normals: nverts*3 or NULL (geometric
+ * normals), albedo_rgb: ntris*3 (base colour, env_map.rchit:36-49). */
+vlb_status  vlb_scene_upload(vlb_ctx* ctx, const float* xyz, uint32_t nverts,
+                             const uint32_t* idx, uint32_t ntris,
+                             const float* normals, const float* albedo_rgb,
+                             uint32_t* out_scene);
+vlb_status  vlb_scene_release(vlb_ctx* ctx, uint32_t scene);
+vlb_status  vlb_scene_info_get(vlb_ctx* ctx, uint32_t scene, vlb_scene_info* out);
+/* Copies the BVH out: nodes as 16 floats each (child boxes + child ids, see
+ * DESIGN), leaf-order triangle ids.  For tests and the traversal byte count. */
+vlb_status  vlb_scene_export_bvh(vlb_ctx* ctx, uint32_t scene, float* nodes,
+                                 size_t nodes_len, int32_t* leaf_prims,
+                                 size_t leaf_len);
+
+/* ------------------------------------------------------------- ray queries */
+/* traceRayEXT closest hit (env_map.rgen:26): prim = -1 on miss; t, u, v are
+ * gl_HitTEXT and hitAttributeEXT (u = weight of v1, v = weight of v2). */
+vlb_status  vlb_trace_closest(vlb_ctx* ctx, uint32_t scene, const float* org,
+                              const float* dir, uint32_t n, float t_min,
+                              float t_max, int32_t* prim, float* t, float* u,
+                              float* v);
+/* Shadow query of env_map.rchit:84-88 (TerminateOnFirstHit | SkipClosestHit):
+ * occluded[i] = 1 if any triangle is hit in [t_min, t_max[i]]. */
+vlb_status  vlb_trace_occluded(vlb_ctx* ctx, uint32_t scene, const float* org,
+                               const float* dir, const float* t_max,
+                               uint32_t n, float t_min, uint8_t* occluded);
+
+/* ---------------------------------------------------------------- baking */
+/* One-shot bake: LightBaker::bake (src/baker/light_baker.cpp:287-328) with
+ * per-probe reset ("intended sum").  out_coeffs: num_probes*K*3 floats. */
+vlb_status  vlb_bake(vlb_ctx* ctx, uint32_t scene, const vlb_bake_params* p,
+                     float* out_coeffs, size_t out_len);
+
+/* Plan API: a prepared, device-resident bake (probe positions, direction and
+ * SH tables, partial buffers).  execute() is asynchronous on the context
+ * stream; sync() waits; download() copies out[P][K][3]. */
+vlb_status  vlb_plan_create(vlb_ctx* ctx, uint32_t scene, const vlb_bake_params* p,
+                            vlb_plan** out);
+vlb_status  vlb_plan_execute(vlb_plan* plan);
+vlb_status  vlb_plan_sync(vlb_plan* plan);
+vlb_status  vlb_plan_download(vlb_plan* plan, float* out_coeffs, size_t out_len);
+vlb_status  vlb_plan_download_f64(vlb_plan* plan, double* out_coeffs, size_t out_len);
+/* Enable per-kernel hipEvent timing in execute (adds event records). */
+vlb_status  vlb_plan_set_timing(vlb_plan* plan, int32_t on);
+/* Capture execute() as a hipGraph and replay it from then on. */
+vlb_status  vlb_plan_use_graph(vlb_plan* plan, int32_t on);
+/* Runs a counting variant of the trace kernel once (node visits, triangle
+ * tests, shadow rays) and fills the counters of vlb_bake_stats. */
+vlb_status  vlb_plan_count(vlb_plan* plan);
+vlb_status  vlb_plan_stats(vlb_plan* plan, vlb_bake_stats* out);
+vlb_status  vlb_plan_destroy(vlb_plan* plan);
+
+/* Per-ray debug output of a bake (EnvMapGenerator::getMap's storage image,
+ * env_map_generator.cpp:325-359): radiance[P][R][3] as fed to the
+ * projection, and the primary hit prim id (-1 = sky) per ray.  Either output
+ * may be NULL. */
+vlb_status  vlb_bake_radiance(vlb_ctx* ctx, uint32_t scene, const vlb_bake_params* p,
+                              float* radiance, size_t radiance_len,
+                              int32_t* hit_prim, size_t hit_len);
+
+/* --------------------------------------------------- SH of images / skies */
+/* Skybox SH of float cubemaps: the build's form of skybox_sh.comp
+ * (shaders/skybox_sh.comp:23-42, host src/skybox_manager.cpp:107-130) for a
+ * 6 x N x N RGBA32F cubemap with exact texel solid angles.  Faces +X,-X,+Y,
+ * -Y,+Z,-Z in Vulkan cube-map orientation.  Host in/out. */
+vlb_status  vlb_skybox_sh(vlb_ctx* ctx, const float* cubemaps, uint32_t face_size,
+                          uint32_t batch, uint32_t sh_order, float* out, size_t out_len);
+/* Device-resident variant: d_cubemaps and d_out are device pointers from
+ * vlb_device_alloc; asynchronous on the context stream.  *kernel_ms (may be
+ * NULL) receives the projection kernel's event-timed duration after a
+ * vlb_ctx_sync. */
+vlb_status  vlb_skybox_sh_d(vlb_ctx* ctx, const void* d_cubemaps, uint32_t face_size,
+                            uint32_t batch, uint32_t sh_order, void* d_out);
+vlb_status  vlb_skybox_last_ms(vlb_ctx* ctx, double* project_ms, double* merge_ms);
+
+/* Projection of an equirect RGBA8 image to 16 SH coefficients x RGB:
+ * variant VLB_EQUIRECT_SH_COMP is shaders/sh.comp:23-42 (host
+ * light_baker.cpp:269-285), VLB_EQUIRECT_SKYBOX_COMP is
+ * shaders/skybox_sh.comp:23-42 (phi - pi/2, SH at dir.xzy).  Intended-sum
+ * semantics (no race, fresh zero).  out: 16*3 floats. */
+vlb_status  vlb_project_equirect(vlb_ctx* ctx, const uint8_t* rgba8, uint32_t width,
+                                 uint32_t height, int32_t variant, float* out16x3);
+
+/* shaders/sh_sum.comp:35-52: per pixel sum_k c_k Y_k(dir) * 1250 stored as
+ * RGBA8 UNORM (alpha 255).  coeffs: 16*3 floats, out: width*height*4. */
+vlb_status  vlb_sh_reconstruct(vlb_ctx* ctx, const float* coeffs16x3, uint32_t width,
+                               uint32_t height, uint8_t* out_rgba8);
+
+/* ------------------------------------------------------- device buffers */
+vlb_status  vlb_device_alloc(vlb_ctx* ctx, size_t bytes, void** dptr);
+vlb_status  vlb_device_free(vlb_ctx* ctx, void* dptr);
+vlb_status  vlb_upload(vlb_ctx* ctx, void* dptr, const void* src, size_t bytes);
+vlb_status  vlb_download(vlb_ctx* ctx, void* dst, const void* dptr, size_t bytes);
+vlb_status  vlb_ctx_sync(vlb_ctx* ctx);
+
+/* ------------------------------------------------------------ host helpers */
+/* LightBaker::probePositionsFromBoudingBox (src/baker/light_baker.cpp:80-101):
+ * counts[0]*counts[1]*counts[2] probes between bmin and bmax, step =
+ * (bmax-bmin)/(counts-1) in FP32.  REFERENCE_BFS reproduces the reference's
+ * breadth-first order and repeated "+= step"; LINEAR is x-fastest with
+ * position bmin + i*step.  out_xyz: P*3 floats, out_step: 3 floats. */
+vlb_status  vlb_probe_grid(const float bmin[3], const float bmax[3],
+                           const uint32_t counts[3], int32_t order,
+                           float* out_xyz, float* out_step);
+/* Direction table used by the bake (float x,y,z per ray). */
+vlb_status  vlb_directions(int32_t dir_mode, uint32_t rays, uint32_t w, uint32_t h,
+                           float* out_xyz, float* out_weight, size_t n);
+/* base64 of light_baker.cpp:330-373 (RFC 4648 alphabet, '=' padding).
+ * Writes at most out_len bytes including the terminating NUL; *written gets
+ * the encoded length.  VLB_ERR_INVALID if out_len is too small. */
+vlb_status  vlb_base64_encode(const uint8_t* bytes, size_t n, char* out,
+                              size_t out_len, size_t* written);
+/* Baked-light glTF of LightBaker::serialize (light_baker.cpp:375-401): the
+ * input glTF JSON text gets the coefficient buffer appended as a base64 data
+ * URI, a bufferView, and light = {gridStep, lmax (= coefficient count),
+ * bufferView}.  Writes at most out_len bytes (incl. NUL); *written gets the
+ * length.  out may be NULL to query the size. */
+vlb_status  vlb_gltf_bake_json(const char* gltf_json, const float* coeffs,
+                               uint32_t num_probes, uint32_t num_coefs,
+                               const float grid_step[3], char* out, size_t out_len,
+                               size_t* written);
+
+/* -------------------------------------------------------- synthetic scenes */
+/* Seeded procedural scenes of the benchmark configurations.  kind:
+ *  0 ROOM    : open-top room 10x4x10 m, 64 boxes U[0.2,1.5] m, 8 icospheres
+ *              subdiv 3 (~11k tris)
+ *  1 TERRAIN : n x n fBm heightfield (4 octaves) + n_spheres icospheres
+ *              subdiv 5 (n=256, 50 spheres: ~1.15M tris)
+ *  2 CITY    : 400 extruded, facade-tessellated buildings (~2M tris)
+ *  3 CUBE    : the 12-triangle cube of default_blender_cube.gltf
+ *  4 SPHERES : n_spheres icospheres subdiv `param` in a box (tests)
+ * The scene lives in library memory until vlb_host_scene_free. */
+typedef struct vlb_host_scene {
+    uint32_t  nverts;
+    uint32_t  ntris;
+    float*    xyz;
+    float*    normals;
+    uint32_t* idx;
+    float*    albedo;
+    float     bounds_min[3];
+    float     bounds_max[3];
+} vlb_host_scene;
+enum { VLB_SCENE_ROOM = 0, VLB_SCENE_TERRAIN = 1, VLB_SCENE_CITY = 2,
+       VLB_SCENE_CUBE = 3, VLB_SCENE_SPHERES = 4 };
+vlb_status  vlb_scene_generate(int32_t kind, uint64_t seed, uint32_t param0,
+                               uint32_t param1, vlb_host_scene** out);
+void        vlb_host_scene_free(vlb_host_scene* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VLB_BAKE_H */

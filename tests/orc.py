@@ -1,0 +1,218 @@
+"""ctypes binding of the CPU oracle (oracle/liborc.so) -- test infrastructure.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+as the checker (or the timed CPU baseline), never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "vulkan-light-bakery_amd"))
+import vlb  # noqa: E402  (for the shared parameter struct layout)
+
+ORC_PATH = os.path.join(ROOT, "oracle", "liborc.so")
+
+
+def _load():
+    if not os.path.exists(ORC_PATH):
+        raise ImportError(f"oracle not built at {ORC_PATH}: run __graft_entry__.build()")
+    L = C.CDLL(ORC_PATH)
+    F, D, U32, I32, VP = C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_uint32, C.c_int32, C.c_void_p
+    sig = {
+        "orc_sh_eval": (None, [I32, I32, D, D]),
+        "orc_fibonacci": (None, [U32, F]),
+        "orc_equirect_dirs": (None, [U32, U32, F, F]),
+        "orc_probe_grid": (None, [F, F, C.POINTER(U32), I32, F, F]),
+        "orc_scene_create": (VP, [F, U32, C.POINTER(U32), U32, F, F, I32]),
+        "orc_scene_free": (None, [VP]),
+        "orc_scene_bvh_depth": (U32, [VP]),
+        "orc_trace_closest_batch": (None, [VP, F, F, U32, C.c_float, C.c_float, I32,
+                                           C.POINTER(I32), F, F, F]),
+        "orc_trace_occluded_batch": (None, [VP, F, F, F, U32, C.c_float, I32, C.POINTER(C.c_uint8)]),
+        "orc_bake": (I32, [VP, C.POINTER(vlb.BakeParams), I32, I32, I32, D, F, C.POINTER(I32),
+                           C.POINTER(C.c_uint64)]),
+        "orc_sky": (None, [C.POINTER(vlb.Sky), D, D]),
+        "orc_skybox_sh": (None, [F, U32, I32, D]),
+        "orc_equirect_project": (None, [C.POINTER(C.c_uint8), U32, U32, I32, D]),
+        "orc_sh_reconstruct": (None, [F, U32, U32, C.POINTER(C.c_uint8)]),
+        "orc_base64": (C.c_size_t, [C.POINTER(C.c_uint8), C.c_size_t, C.c_char_p]),
+        "orc_srgb": (C.c_float, [C.c_float]),
+        "orc_unorm8": (C.c_float, [C.c_float]),
+    }
+    for n, (r, a) in sig.items():
+        f = getattr(L, n)
+        f.restype = r
+        f.argtypes = a
+    return L
+
+
+L = _load()
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def sh_eval(dirs, K=16, exact=True) -> np.ndarray:
+    d = np.ascontiguousarray(np.asarray(dirs, np.float64).reshape(-1, 3))
+    out = np.zeros((d.shape[0], K), np.float64)
+    for i in range(d.shape[0]):
+        L.orc_sh_eval(K, int(exact), _d(d[i]), _d(out[i]))
+    return out
+
+
+def fibonacci(R: int) -> np.ndarray:
+    out = np.zeros((R, 3), np.float32)
+    L.orc_fibonacci(R, _f(out))
+    return out
+
+
+def equirect_dirs(W: int, H: int):
+    xyz = np.zeros((W * H, 3), np.float32)
+    w = np.zeros(W * H, np.float32)
+    L.orc_equirect_dirs(W, H, _f(xyz), _f(w))
+    return xyz, w
+
+
+def probe_grid(b0, b1, counts, order):
+    b0, b1 = f32(b0), f32(b1)
+    c = np.ascontiguousarray(np.asarray(counts, np.uint32))
+    out = np.zeros((int(np.prod(c.astype(np.int64))), 3), np.float32)
+    step = np.zeros(3, np.float32)
+    L.orc_probe_grid(_f(b0), _f(b1), c.ctypes.data_as(C.POINTER(C.c_uint32)), order, _f(out), _f(step))
+    return out, step
+
+
+class Scene:
+    def __init__(self, xyz, idx, normals=None, albedo=None, bvh=True):
+        self.xyz = f32(xyz).reshape(-1, 3)
+        self.idx = np.ascontiguousarray(np.asarray(idx, np.uint32).reshape(-1, 3))
+        self.nrm = None if normals is None else f32(normals).reshape(-1, 3)
+        self.alb = None if albedo is None else f32(albedo).reshape(-1, 3)
+        self.h = L.orc_scene_create(_f(self.xyz), self.xyz.shape[0],
+                                    self.idx.ctypes.data_as(C.POINTER(C.c_uint32)), self.idx.shape[0],
+                                    None if self.nrm is None else _f(self.nrm),
+                                    None if self.alb is None else _f(self.alb), int(bvh))
+
+    @classmethod
+    def from_dict(cls, s, bvh=True):
+        return cls(s["xyz"], s["idx"], s.get("normals"), s.get("albedo"), bvh)
+
+    def depth(self):
+        return L.orc_scene_bvh_depth(self.h)
+
+    def trace_closest(self, org, dirs, t_min=0.001, t_max=10000.0, brute=False):
+        org, dirs = f32(org).reshape(-1, 3), f32(dirs).reshape(-1, 3)
+        n = org.shape[0]
+        prim = np.zeros(n, np.int32)
+        t, u, v = (np.zeros(n, np.float32) for _ in range(3))
+        L.orc_trace_closest_batch(self.h, _f(org), _f(dirs), n, t_min, t_max, int(brute),
+                                  prim.ctypes.data_as(C.POINTER(C.c_int32)), _f(t), _f(u), _f(v))
+        return prim, t, u, v
+
+    def trace_occluded(self, org, dirs, t_max, t_min=0.0, brute=False):
+        org, dirs = f32(org).reshape(-1, 3), f32(dirs).reshape(-1, 3)
+        n = org.shape[0]
+        tm = f32(np.broadcast_to(np.asarray(t_max, np.float32), (n,)))
+        occ = np.zeros(n, np.uint8)
+        L.orc_trace_occluded_batch(self.h, _f(org), _f(dirs), _f(tm), n, t_min, int(brute),
+                                   occ.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return occ
+
+    def bake(self, params: "vlb.Params", mode64=True, nthreads=0, brute=False,
+             want_radiance=False, want_hits=False):
+        p = params.to_c()
+        P, K, R = p.num_probes, params.K, params.R
+        out = np.zeros((P, K, 3), np.float64)
+        rad = np.zeros((P, R, 3), np.float32) if want_radiance else None
+        hit = np.zeros((P, R), np.int32) if want_hits else None
+        nsh = C.c_uint64(0)
+        st = L.orc_bake(self.h, C.byref(p), int(mode64), nthreads, int(brute), _d(out),
+                        None if rad is None else _f(rad),
+                        None if hit is None else hit.ctypes.data_as(C.POINTER(C.c_int32)),
+                        C.byref(nsh))
+        if st:
+            raise ValueError("orc_bake rejected the parameters")
+        return {"coeffs": out, "radiance": rad, "hits": hit, "shadow_rays": nsh.value}
+
+    def __del__(self):
+        try:
+            if self.h:
+                L.orc_scene_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def skybox_sh(cube, K=9) -> np.ndarray:
+    c = f32(cube)
+    N = c.shape[1]
+    out = np.zeros((K, 3), np.float64)
+    L.orc_skybox_sh(_f(c), N, K, _d(out))
+    return out
+
+
+def equirect_project(img, variant) -> np.ndarray:
+    im = np.ascontiguousarray(img, dtype=np.uint8)
+    H, W, _ = im.shape
+    out = np.zeros((16, 3), np.float64)
+    L.orc_equirect_project(im.ctypes.data_as(C.POINTER(C.c_uint8)), W, H, variant, _d(out))
+    return out
+
+
+def sh_reconstruct(coeffs, W, H) -> np.ndarray:
+    c = f32(coeffs).reshape(16, 3)
+    out = np.zeros((H, W, 4), np.uint8)
+    L.orc_sh_reconstruct(_f(c), W, H, out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out
+
+
+def base64(data: bytes) -> str:
+    n = len(data)
+    buf = (C.c_uint8 * max(1, n)).from_buffer_copy(data or b"\0")
+    out = C.create_string_buffer(4 * ((n + 2) // 3) + 1)
+    L.orc_base64(buf, n, out)
+    return out.value.decode()
+
+
+def sky(params: "vlb.Params", dirs) -> np.ndarray:
+    p = params.to_c()
+    d = np.ascontiguousarray(np.asarray(dirs, np.float64).reshape(-1, 3))
+    out = np.zeros_like(d)
+    for i in range(d.shape[0]):
+        L.orc_sky(C.byref(p.sky), _d(d[i]), _d(out[i]))
+    return out
+
+
+def procedural_cubemap(N: int, sun, zenith=2.0, horizon=0.5, sun_rgb=50.0, k=200.0,
+                       tint=(1.0, 1.0, 1.0)) -> np.ndarray:
+    """6 x N x N RGBA32F cubemap of the configuration's sky (Vulkan faces)."""
+    c = (np.arange(N) + 0.5) * (2.0 / N) - 1.0
+    v, u = np.meshgrid(c, c, indexing="ij")
+    one = np.ones_like(u)
+    faces = [(one, -v, -u), (-one, -v, u), (u, one, v), (u, -one, -v), (u, -v, one), (-u, -v, -one)]
+    out = np.zeros((6, N, N, 4), np.float32)
+    s = np.asarray(sun, np.float64)
+    for f, (x, y, z) in enumerate(faces):
+        l = np.sqrt(x * x + y * y + z * z)
+        x, y, z = x / l, y / l, z / l
+        g = np.maximum(y, 0.0)
+        base = horizon + (zenith - horizon) * g
+        lobe = sun_rgb * np.exp(k * (x * s[0] + y * s[1] + z * s[2] - 1.0))
+        for ch in range(3):
+            out[f, :, :, ch] = (base + lobe) * tint[ch]
+        out[f, :, :, 3] = 1.0
+    return out

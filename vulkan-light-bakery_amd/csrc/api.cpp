@@ -1,0 +1,955 @@
+// api.cpp -- the C ABI of include/vlb_bake.h over the HIP kernels.
+//
+// Host-side replacement of the reference's Vulkan plumbing: Application
+// (src/application.cpp:540-556) becomes one HIP device + one stream;
+// LightBaker's per-probe submit/fence loop (src/baker/light_baker.cpp:287-328)
+// becomes one asynchronous plan execution (bake kernel + FP64 merge).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "vlb_internal.hpp"
+
+using namespace vlb;
+
+namespace vlb {
+
+DevBuf dev_alloc(Context& ctx, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    const uint64_t now = ctx.in_use.load();
+    if (ctx.budget && now + bytes > ctx.budget)
+        throw Error{VLB_ERR_BUDGET, "request of " + std::to_string(bytes) +
+                                        " device bytes exceeds the HBM budget (" +
+                                        std::to_string(now) + " of " + std::to_string(ctx.budget) +
+                                        " in use)"};
+    DevBuf b;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{VLB_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e)};
+    }
+    b.bytes = bytes;
+    b.acct = &ctx.in_use;
+    ctx.in_use.fetch_add(bytes);
+    return b;
+}
+
+Plan::~Plan()
+{
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+}
+
+}  // namespace vlb
+
+namespace {
+
+template <class F>
+vlb_status guard(Context* ctx, F&& f)
+{
+    try {
+        f();
+        return VLB_OK;
+    } catch (const Error& e) {
+        if (ctx) ctx->last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->last_error = "host allocation failed";
+        return VLB_ERR_OOM;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->last_error = e.what();
+        return VLB_ERR_INVALID;
+    } catch (...) {
+        if (ctx) ctx->last_error = "unknown error";
+        return VLB_ERR_INVALID;
+    }
+}
+
+Scene& find_scene(Context& ctx, uint32_t id)
+{
+    auto it = ctx.scenes.find(id);
+    if (it == ctx.scenes.end()) throw Error{VLB_ERR_NOT_FOUND, "unknown scene id " + std::to_string(id)};
+    return *it->second;
+}
+
+template <class T>
+DevBuf upload_vec(Context& ctx, const std::vector<T>& v)
+{
+    DevBuf b = dev_alloc(ctx, v.size() * sizeof(T));
+    VLB_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx.stream));
+    return b;
+}
+
+uint32_t expand10(uint32_t v)
+{
+    v &= 1023u;
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// Morton order of 3D points (stable on equal codes).
+std::vector<uint32_t> morton_order(const float* xyz, size_t n)
+{
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], xyz[3 * i + a]);
+            hi[a] = std::max(hi[a], xyz[3 * i + a]);
+        }
+    std::vector<uint32_t> code(n), order(n);
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t q[3];
+        for (int a = 0; a < 3; ++a) {
+            float ext = hi[a] - lo[a];
+            float x = ext > 0.0f ? (xyz[3 * i + a] - lo[a]) / ext * 1023.0f : 0.0f;
+            q[a] = (uint32_t)std::min(1023.0f, std::max(0.0f, x));
+        }
+        code[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+    }
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
+    return order;
+}
+
+uint16_t to_bf16(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)(u >> 16);  // inf / nan
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+uint16_t to_fp16(float f)
+{
+    _Float16 h = (_Float16)f;
+    uint16_t u;
+    memcpy(&u, &h, 2);
+    return u;
+}
+
+void validate(const vlb_bake_params& p)
+{
+    VLB_REQUIRE(p.num_probes >= 1 && p.probe_xyz, VLB_ERR_INVALID, "num_probes must be >= 1 with probe_xyz");
+    VLB_REQUIRE(p.num_probes <= (1u << 26), VLB_ERR_INVALID, "num_probes too large");
+    VLB_REQUIRE(p.sh_order == 2 || p.sh_order == 3, VLB_ERR_INVALID, "sh_order must be 2 or 3");
+    VLB_REQUIRE(p.sh_constants == VLB_SH_LITERAL || p.sh_constants == VLB_SH_EXACT, VLB_ERR_INVALID,
+                "bad sh_constants");
+    VLB_REQUIRE(p.precision >= VLB_PREC_FP32 && p.precision <= VLB_PREC_FP16, VLB_ERR_INVALID,
+                "bad precision");
+    VLB_REQUIRE(p.shade_mode == VLB_SHADE_LAMBERT || p.shade_mode == VLB_SHADE_REFERENCE,
+                VLB_ERR_INVALID, "bad shade_mode");
+    VLB_REQUIRE(p.num_lights <= VLB_MAX_LIGHTS, VLB_ERR_INVALID, "too many lights");
+    VLB_REQUIRE(p.num_lights == 0 || p.lights, VLB_ERR_INVALID, "lights pointer is NULL");
+    for (uint32_t i = 0; i < p.num_lights; ++i)
+        VLB_REQUIRE(p.lights[i].type == VLB_LIGHT_DIRECTIONAL || p.lights[i].type == VLB_LIGHT_POINT,
+                    VLB_ERR_INVALID, "bad light type");
+    if (p.dir_mode == VLB_DIRS_FIBONACCI) {
+        VLB_REQUIRE(p.rays_per_probe >= 1 && p.rays_per_probe <= (1u << 26), VLB_ERR_INVALID,
+                    "rays_per_probe out of range");
+    } else if (p.dir_mode == VLB_DIRS_EQUIRECT) {
+        VLB_REQUIRE(p.equirect_w >= 1 && p.equirect_h >= 1 &&
+                        (uint64_t)p.equirect_w * p.equirect_h <= (1u << 26),
+                    VLB_ERR_INVALID, "equirect size out of range");
+    } else {
+        throw Error{VLB_ERR_INVALID, "bad dir_mode"};
+    }
+    VLB_REQUIRE(p.t_min >= 0.0f && p.t_max > p.t_min, VLB_ERR_INVALID, "need 0 <= t_min < t_max");
+    VLB_REQUIRE((uint64_t)p.num_probes * 48ull * 8ull < (1ull << 40), VLB_ERR_INVALID, "bake too large");
+}
+
+void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
+{
+    validate(p);
+    pl.ctx = &ctx;
+    pl.scene = &sc;
+    pl.params = p;
+    pl.lights.assign(p.lights, p.lights + p.num_lights);
+    pl.params.lights = pl.lights.data();
+    pl.params.probe_xyz = nullptr;
+    pl.K = p.sh_order == 2 ? 9 : 16;
+    pl.precision = p.precision;
+    const int K = pl.K;
+    const uint32_t P = p.num_probes;
+
+    // ---- direction set (host, deterministic; same formulas as the oracle)
+    uint32_t R;
+    std::vector<float> dxyz, dw;
+    if (p.dir_mode == VLB_DIRS_FIBONACCI) {
+        R = p.rays_per_probe;
+        dxyz.resize((size_t)R * 3);
+        dw.assign(R, (float)(4.0 * M_PI / R));
+        fibonacci_dirs(R, dxyz.data());
+    } else {
+        R = p.equirect_w * p.equirect_h;
+        dxyz.resize((size_t)R * 3);
+        dw.resize(R);
+        equirect_dirs(p.equirect_w, p.equirect_h, dxyz.data(), dw.data());
+    }
+    const double wref = 4.0 * M_PI / R;
+    // spatial order of directions: neighbouring lanes trace neighbouring rays
+    const std::vector<uint32_t> dorder = morton_order(dxyz.data(), R);
+
+    // ---- tiling
+    pl.grid = bake_grid_size(ctx, p.precision);
+    const int npg = (int)((P + 15) / 16);
+    const int Ppad = npg * 16;
+    uint32_t RS;
+    if (p.rays_per_slice) {
+        RS = (p.rays_per_slice + 31) & ~31u;
+    } else {
+        uint64_t nsl = ((uint64_t)4 * pl.grid + npg - 1) / npg;
+        const uint64_t max_sl = (R + 31) / 32;
+        nsl = std::max<uint64_t>(1, std::min(nsl, max_sl));
+        RS = (uint32_t)(((R + nsl - 1) / nsl + 31) & ~31ull);
+    }
+    const int nslices = (int)((R + RS - 1) / RS);
+    const int Rpad = nslices * (int)RS;
+    VLB_REQUIRE((uint64_t)npg * nslices < (1ull << 31), VLB_ERR_INVALID, "too many tiles");
+
+    // ---- direction + SH tables in sorted order
+    std::vector<float4> dirs4(Rpad, make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<int> dperm(Rpad, 0);
+    std::vector<float> yt32;
+    std::vector<uint16_t> yt16;
+    if (p.precision == VLB_PREC_FP32) yt32.assign((size_t)Rpad * 16, 0.0f);
+    else yt16.assign((size_t)Rpad * 16, 0);
+    const bool exact = p.sh_constants == VLB_SH_EXACT;
+    for (int r = 0; r < Rpad; ++r) {
+        if (r >= (int)R) continue;
+        const uint32_t c = dorder[r];
+        const float* d = &dxyz[3 * (size_t)c];
+        dirs4[r] = make_float4(d[0], d[1], d[2], 1.0f);
+        dperm[r] = (int)c;
+        double sh[3] = {d[0], d[2], d[1]};  // SH frame: world (x, z, y)
+        double Y[16];
+        sh_eval_double(K, exact, sh, Y);
+        const double wr = (double)dw[c] / wref;
+        for (int k = 0; k < K; ++k) {
+            const size_t ti = ((size_t)(r >> 3) * 16 + k) * 8 + (r & 7);
+            const float v = (float)(Y[k] * wr);
+            if (p.precision == VLB_PREC_FP32) yt32[ti] = v;
+            else if (p.precision == VLB_PREC_BF16) yt16[ti] = to_bf16(v);
+            else yt16[ti] = to_fp16(v);
+        }
+    }
+    // ---- probes in Morton order, padded to groups of 16
+    const std::vector<uint32_t> porder = morton_order(p.probe_xyz, P);
+    std::vector<float4> probes4(Ppad, make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<int> pperm(Ppad, -1);
+    for (uint32_t i = 0; i < P; ++i) {
+        const float* q = &p.probe_xyz[3 * (size_t)porder[i]];
+        probes4[i] = make_float4(q[0], q[1], q[2], 1.0f);
+        pperm[i] = (int)porder[i];
+    }
+    for (int i = (int)P; i < Ppad; ++i) probes4[i] = probes4[P - 1];
+
+    // ---- device buffers
+    pl.probes = upload_vec(ctx, probes4);
+    pl.probe_perm = upload_vec(ctx, pperm);
+    pl.dirs = upload_vec(ctx, dirs4);
+    pl.dir_perm = upload_vec(ctx, dperm);
+    if (p.precision == VLB_PREC_FP32) pl.ytab = upload_vec(ctx, yt32);
+    else pl.ytab = upload_vec(ctx, yt16);
+    pl.partials = dev_alloc(ctx, (size_t)nslices * Ppad * 48 * sizeof(float2));
+    const uint32_t spill_stride = (uint32_t)pl.grid * 256u;
+    const uint32_t spill_entries = sc.depth > (uint32_t)kLdsDepth ? sc.depth - kLdsDepth + 1 : 0;
+    pl.spill = dev_alloc(ctx, std::max<size_t>(16, (size_t)spill_entries * spill_stride * 4));
+    pl.out = dev_alloc(ctx, (size_t)P * K * 3 * sizeof(float));
+    pl.out64 = dev_alloc(ctx, (size_t)P * K * 3 * sizeof(double));
+    pl.counters = dev_alloc(ctx, 4 * sizeof(unsigned long long));
+
+    // ---- kernel arguments
+    BakeArgs& a = pl.args;
+    memset(&a, 0, sizeof a);
+    a.sc = sc.view();
+    a.sp.mode = p.shade_mode;
+    a.sp.nlights = (int)p.num_lights;
+    a.sp.shadow_bias = p.shadow_bias;
+    a.sp.t_max = p.t_max;
+    for (uint32_t i = 0; i < p.num_lights; ++i) a.sp.lights[i] = p.lights[i];
+    a.sp.sky = p.sky;
+    a.sp.ref = p.ref;
+    a.probes = pl.probes.as<float4>();
+    a.P = (int)P;
+    a.Ppad = Ppad;
+    a.dirs = pl.dirs.as<float4>();
+    a.ytab = pl.ytab.p;
+    a.R = (int)R;
+    a.Rpad = Rpad;
+    a.RS = (int)RS;
+    a.nslices = nslices;
+    a.npg = npg;
+    a.ntiles = npg * nslices;
+    a.tmin = p.t_min;
+    a.tmax = p.t_max;
+    a.partials = pl.partials.as<float2>();
+    a.spill = pl.spill.as<uint32_t>();
+    a.spill_stride = spill_stride;
+    a.probe_perm = pl.probe_perm.as<int>();
+    a.dir_perm = pl.dir_perm.as<int>();
+    a.counters = pl.counters.as<unsigned long long>();
+
+    MergeArgs& m = pl.margs;
+    m.partials = pl.partials.as<float2>();
+    m.nslices = nslices;
+    m.P = (int)P;
+    m.Ppad = Ppad;
+    m.K = K;
+    m.probe_perm = pl.probe_perm.as<int>();
+    m.scale = wref;
+    m.out = pl.out.as<float>();
+    m.out64 = pl.out64.as<double>();
+
+    pl.stats.primary_rays = (uint64_t)P * R;
+    pl.stats.tiles = (uint32_t)a.ntiles;
+    pl.stats.grid = (uint32_t)pl.grid;
+    pl.stats.rays_per_slice = RS;
+    pl.stats.slices = (uint32_t)nslices;
+    for (auto& e : pl.ev) VLB_HIP(hipEventCreate(&e));
+    VLB_HIP(hipStreamSynchronize(ctx.stream));
+}
+
+void plan_launch(Plan& pl)
+{
+    hipStream_t st = pl.ctx->stream;
+    if (pl.use_graph) {
+        if (!pl.graph_exec) {
+            VLB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                launch_bake(pl.args, pl.precision, false, false, pl.grid, st);
+                launch_merge(pl.margs, st);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            VLB_HIP(hipStreamEndCapture(st, &pl.graph));
+            VLB_HIP(hipGraphInstantiate(&pl.graph_exec, pl.graph, nullptr, nullptr, 0));
+        }
+        if (pl.timing) VLB_HIP(hipEventRecord(pl.ev[0], st));
+        VLB_HIP(hipGraphLaunch(pl.graph_exec, st));
+        if (pl.timing) VLB_HIP(hipEventRecord(pl.ev[2], st));
+    } else {
+        if (pl.timing) VLB_HIP(hipEventRecord(pl.ev[0], st));
+        launch_bake(pl.args, pl.precision, false, false, pl.grid, st);
+        if (pl.timing) VLB_HIP(hipEventRecord(pl.ev[1], st));
+        launch_merge(pl.margs, st);
+        if (pl.timing) VLB_HIP(hipEventRecord(pl.ev[2], st));
+    }
+    pl.ran = true;
+}
+
+void plan_sync(Plan& pl)
+{
+    VLB_HIP(hipStreamSynchronize(pl.ctx->stream));
+    if (pl.timing && pl.ran) {
+        float ms = 0.f;
+        VLB_HIP(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[2]));
+        pl.stats.total_ms = ms;
+        if (!pl.use_graph) {
+            VLB_HIP(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[1]));
+            pl.stats.trace_ms = ms;
+            VLB_HIP(hipEventElapsedTime(&ms, pl.ev[1], pl.ev[2]));
+            pl.stats.merge_ms = ms;
+        }
+    }
+}
+
+size_t scene_spill_bytes(const Scene& s, uint32_t stride)
+{
+    const uint32_t e = s.depth > (uint32_t)kLdsDepth ? s.depth - kLdsDepth + 1 : 0;
+    return std::max<size_t>(16, (size_t)e * stride * 4);
+}
+
+std::vector<float> solid_angle_table(int N)
+{
+    auto A = [](double x, double y) { return atan2(x * y, sqrt(x * x + y * y + 1.0)); };
+    std::vector<float> w((size_t)N * N);
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i) {
+            double x0 = 2.0 * i / N - 1.0, x1 = 2.0 * (i + 1) / N - 1.0;
+            double y0 = 2.0 * j / N - 1.0, y1 = 2.0 * (j + 1) / N - 1.0;
+            w[(size_t)j * N + i] = (float)(A(x0, y0) - A(x0, y1) - A(x1, y0) + A(x1, y1));
+        }
+    return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t vlb_abi_version(void) { return VLB_ABI_VERSION; }
+
+vlb_status vlb_ctx_create(int32_t device, uint64_t hbm_budget_bytes, vlb_ctx** out)
+{
+    if (!out) return VLB_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return VLB_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) return VLB_ERR_INVALID;
+    Context* c = new Context();
+    vlb_status s = guard(c, [&] {
+        c->device = device;
+        VLB_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        VLB_HIP(hipGetDeviceProperties(&prop, device));
+        c->num_cus = prop.multiProcessorCount;
+        c->name = prop.gcnArchName;
+        size_t freeb = 0, total = 0;
+        VLB_HIP(hipMemGetInfo(&freeb, &total));
+        c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)freeb;
+        VLB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto& e : c->ev) VLB_HIP(hipEventCreate(&e));
+    });
+    if (s != VLB_OK) {
+        delete c;
+        return s;
+    }
+    *out = reinterpret_cast<vlb_ctx*>(c);
+    return VLB_OK;
+}
+
+vlb_status vlb_ctx_destroy(vlb_ctx* h)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->scenes.clear();
+    c->user_bufs.clear();
+    c->wtab.release();
+    c->sky_partial.release();
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return VLB_OK;
+}
+
+const char* vlb_last_error(const vlb_ctx* h)
+{
+    if (!h) return "null context";
+    return reinterpret_cast<const Context*>(h)->last_error.c_str();
+}
+
+vlb_status vlb_device_info(vlb_ctx* h, int32_t* num_cus, char* name, size_t name_len)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    if (num_cus) *num_cus = c->num_cus;
+    if (name && name_len) {
+        strncpy(name, c->name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return VLB_OK;
+}
+
+// ------------------------------------------------------------------ scenes
+vlb_status vlb_scene_upload(vlb_ctx* h, const float* xyz, uint32_t nverts, const uint32_t* idx,
+                            uint32_t ntris, const float* normals, const float* albedo,
+                            uint32_t* out_scene)
+{
+    if (!h || !out_scene) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_REQUIRE(ntris == 0 || (xyz && idx && nverts > 0), VLB_ERR_INVALID, "missing geometry");
+        VLB_REQUIRE(ntris < (1u << 30), VLB_ERR_INVALID, "too many triangles");
+        for (uint64_t i = 0; i < (uint64_t)ntris * 3; ++i)
+            VLB_REQUIRE(idx[i] < nverts, VLB_ERR_INVALID, "index out of range at " + std::to_string(i));
+        VLB_HIP(hipSetDevice(c->device));
+        auto s = std::make_unique<Scene>();
+        build_scene_bvh(*c, *s, xyz, nverts, idx, ntris, normals, albedo);
+        VLB_REQUIRE(s->depth <= 4096, VLB_ERR_INVALID, "BVH build produced a malformed tree");
+        s->id = c->next_scene++;
+        *out_scene = s->id;
+        c->scenes[s->id] = std::move(s);
+    });
+}
+
+vlb_status vlb_scene_release(vlb_ctx* h, uint32_t scene)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        find_scene(*c, scene);
+        VLB_HIP(hipStreamSynchronize(c->stream));
+        c->scenes.erase(scene);
+    });
+}
+
+vlb_status vlb_scene_info_get(vlb_ctx* h, uint32_t scene, vlb_scene_info* out)
+{
+    if (!h || !out) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        Scene& s = find_scene(*c, scene);
+        memset(out, 0, sizeof *out);
+        out->num_tris = (uint32_t)s.ntris;
+        out->num_nodes = (uint32_t)s.nnodes;
+        out->depth = s.depth;
+        out->device_bytes = s.bytes();
+        out->build_ms = s.build_ms;
+        for (int a = 0; a < 3; ++a) {
+            out->bounds_min[a] = s.bmin[a];
+            out->bounds_max[a] = s.bmax[a];
+        }
+    });
+}
+
+vlb_status vlb_scene_export_bvh(vlb_ctx* h, uint32_t scene, float* nodes, size_t nodes_len,
+                                int32_t* leaf_prims, size_t leaf_len)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        Scene& s = find_scene(*c, scene);
+        VLB_HIP(hipSetDevice(c->device));
+        if (nodes) {
+            VLB_REQUIRE(nodes_len >= (size_t)s.nnodes * 16, VLB_ERR_INVALID, "nodes buffer too small");
+            VLB_HIP(hipMemcpyAsync(nodes, s.nodes.p, (size_t)s.nnodes * sizeof(BNode),
+                                   hipMemcpyDeviceToHost, c->stream));
+        }
+        if (leaf_prims) {
+            VLB_REQUIRE(leaf_len >= (size_t)s.ntris, VLB_ERR_INVALID, "leaf buffer too small");
+            std::vector<float4> t((size_t)s.ntris * 3);
+            VLB_HIP(hipMemcpyAsync(t.data(), s.tris.p, t.size() * sizeof(float4), hipMemcpyDeviceToHost,
+                                   c->stream));
+            VLB_HIP(hipStreamSynchronize(c->stream));
+            for (int i = 0; i < s.ntris; ++i) {
+                int32_t v;
+                memcpy(&v, &t[3 * (size_t)i].w, 4);
+                leaf_prims[i] = v;
+            }
+        }
+        VLB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+// ------------------------------------------------------------- ray queries
+vlb_status vlb_trace_closest(vlb_ctx* h, uint32_t scene, const float* org, const float* dir,
+                             uint32_t n, float t_min, float t_max, int32_t* prim, float* t,
+                             float* u, float* v)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_REQUIRE(org && dir && prim && t && u && v, VLB_ERR_INVALID, "null buffer");
+        Scene& s = find_scene(*c, scene);
+        VLB_HIP(hipSetDevice(c->device));
+        if (n == 0) return;
+        const int grid = std::max(8, c->num_cus * 4);
+        const uint32_t stride = (uint32_t)grid * 256u;
+        DevBuf d_o = dev_alloc(*c, (size_t)n * 12), d_d = dev_alloc(*c, (size_t)n * 12);
+        DevBuf d_p = dev_alloc(*c, (size_t)n * 4), d_t = dev_alloc(*c, (size_t)n * 4);
+        DevBuf d_u = dev_alloc(*c, (size_t)n * 4), d_v = dev_alloc(*c, (size_t)n * 4);
+        DevBuf spill = dev_alloc(*c, scene_spill_bytes(s, stride));
+        VLB_HIP(hipMemcpyAsync(d_o.p, org, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+        VLB_HIP(hipMemcpyAsync(d_d.p, dir, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+        launch_trace_closest(s.view(), d_o.as<float>(), d_d.as<float>(), (int)n, t_min, t_max,
+                             d_p.as<int>(), d_t.as<float>(), d_u.as<float>(), d_v.as<float>(),
+                             spill.as<uint32_t>(), stride, grid, c->stream);
+        VLB_HIP(hipMemcpyAsync(prim, d_p.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+        VLB_HIP(hipMemcpyAsync(t, d_t.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+        VLB_HIP(hipMemcpyAsync(u, d_u.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+        VLB_HIP(hipMemcpyAsync(v, d_v.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+        VLB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+vlb_status vlb_trace_occluded(vlb_ctx* h, uint32_t scene, const float* org, const float* dir,
+                              const float* t_max, uint32_t n, float t_min, uint8_t* occluded)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_REQUIRE(org && dir && t_max && occluded, VLB_ERR_INVALID, "null buffer");
+        Scene& s = find_scene(*c, scene);
+        VLB_HIP(hipSetDevice(c->device));
+        if (n == 0) return;
+        const int grid = std::max(8, c->num_cus * 4);
+        const uint32_t stride = (uint32_t)grid * 256u;
+        DevBuf d_o = dev_alloc(*c, (size_t)n * 12), d_d = dev_alloc(*c, (size_t)n * 12);
+        DevBuf d_m = dev_alloc(*c, (size_t)n * 4), d_r = dev_alloc(*c, (size_t)n);
+        DevBuf spill = dev_alloc(*c, scene_spill_bytes(s, stride));
+        VLB_HIP(hipMemcpyAsync(d_o.p, org, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+        VLB_HIP(hipMemcpyAsync(d_d.p, dir, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+        VLB_HIP(hipMemcpyAsync(d_m.p, t_max, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+        launch_trace_occluded(s.view(), d_o.as<float>(), d_d.as<float>(), d_m.as<float>(), (int)n,
+                              t_min, d_r.as<uint8_t>(), spill.as<uint32_t>(), stride, grid, c->stream);
+        VLB_HIP(hipMemcpyAsync(occluded, d_r.p, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+        VLB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+// ------------------------------------------------------------------ plans
+vlb_status vlb_plan_create(vlb_ctx* h, uint32_t scene, const vlb_bake_params* p, vlb_plan** out)
+{
+    if (!h || !p || !out) return VLB_ERR_INVALID;
+    *out = nullptr;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        Scene& s = find_scene(*c, scene);
+        VLB_HIP(hipSetDevice(c->device));
+        auto pl = std::make_unique<Plan>();
+        build_plan(*c, s, *p, *pl);
+        *out = reinterpret_cast<vlb_plan*>(pl.release());
+    });
+}
+
+vlb_status vlb_plan_execute(vlb_plan* h)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Plan* pl = reinterpret_cast<Plan*>(h);
+    std::lock_guard<std::recursive_mutex> lk(pl->ctx->mtx);
+    return guard(pl->ctx, [&] { plan_launch(*pl); });
+}
+
+vlb_status vlb_plan_sync(vlb_plan* h)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Plan* pl = reinterpret_cast<Plan*>(h);
+    std::lock_guard<std::recursive_mutex> lk(pl->ctx->mtx);
+    return guard(pl->ctx, [&] { plan_sync(*pl); });
+}
+
+vlb_status vlb_plan_download(vlb_plan* h, float* out, size_t out_len)
+{
+    if (!h || !out) return VLB_ERR_INVALID;
+    Plan* pl = reinterpret_cast<Plan*>(h);
+    std::lock_guard<std::recursive_mutex> lk(pl->ctx->mtx);
+    return guard(pl->ctx, [&] {
+        const size_t n = (size_t)pl->args.P * pl->K * 3;
+        VLB_REQUIRE(out_len >= n, VLB_ERR_INVALID, "output buffer too small");
+        VLB_REQUIRE(pl->ran, VLB_ERR_INVALID, "plan has not been executed");
+        VLB_HIP(hipMemcpyAsync(out, pl->out.p, n * sizeof(float), hipMemcpyDeviceToHost, pl->ctx->stream));
+        plan_sync(*pl);
+    });
+}
+
+vlb_status vlb_plan_download_f64(vlb_plan* h, double* out, size_t out_len)
+{
+    if (!h || !out) return VLB_ERR_INVALID;
+    Plan* pl = reinterpret_cast<Plan*>(h);
+    std::lock_guard<std::recursive_mutex> lk(pl->ctx->mtx);
+    return guard(pl->ctx, [&] {
+        const size_t n = (size_t)pl->args.P * pl->K * 3;
+        VLB_REQUIRE(out_len >= n, VLB_ERR_INVALID, "output buffer too small");
+        VLB_REQUIRE(pl->ran, VLB_ERR_INVALID, "plan has not been executed");
+        VLB_HIP(hipMemcpyAsync(out, pl->out64.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                               pl->ctx->stream));
+        plan_sync(*pl);
+    });
+}
+
+vlb_status vlb_plan_set_timing(vlb_plan* h, int32_t on)
+{
+    if (!h) return VLB_ERR_INVALID;
+    reinterpret_cast<Plan*>(h)->timing = on != 0;
+    return VLB_OK;
+}
+
+vlb_status vlb_plan_use_graph(vlb_plan* h, int32_t on)
+{
+    if (!h) return VLB_ERR_INVALID;
+    reinterpret_cast<Plan*>(h)->use_graph = on != 0;
+    return VLB_OK;
+}
+
+vlb_status vlb_plan_count(vlb_plan* h)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Plan* pl = reinterpret_cast<Plan*>(h);
+    std::lock_guard<std::recursive_mutex> lk(pl->ctx->mtx);
+    return guard(pl->ctx, [&] {
+        hipStream_t st = pl->ctx->stream;
+        VLB_HIP(hipMemsetAsync(pl->counters.p, 0, 4 * sizeof(unsigned long long), st));
+        launch_bake(pl->args, VLB_PREC_FP32, false, true, pl->grid, st);
+        unsigned long long cnt[4] = {0, 0, 0, 0};
+        VLB_HIP(hipMemcpyAsync(cnt, pl->counters.p, sizeof cnt, hipMemcpyDeviceToHost, st));
+        VLB_HIP(hipStreamSynchronize(st));
+        pl->stats.node_visits = cnt[0];
+        pl->stats.tri_tests = cnt[1];
+        pl->stats.shadow_rays = cnt[2];
+    });
+}
+
+vlb_status vlb_plan_stats(vlb_plan* h, vlb_bake_stats* out)
+{
+    if (!h || !out) return VLB_ERR_INVALID;
+    *out = reinterpret_cast<Plan*>(h)->stats;
+    return VLB_OK;
+}
+
+vlb_status vlb_plan_destroy(vlb_plan* h)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Plan* pl = reinterpret_cast<Plan*>(h);
+    Context* c = pl->ctx;
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    (void)hipStreamSynchronize(c->stream);
+    delete pl;
+    return VLB_OK;
+}
+
+vlb_status vlb_bake(vlb_ctx* h, uint32_t scene, const vlb_bake_params* p, float* out, size_t out_len)
+{
+    if (!h || !p || !out) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        Scene& s = find_scene(*c, scene);
+        VLB_HIP(hipSetDevice(c->device));
+        Plan pl;
+        build_plan(*c, s, *p, pl);
+        const size_t n = (size_t)p->num_probes * pl.K * 3;
+        VLB_REQUIRE(out_len >= n, VLB_ERR_INVALID, "output buffer too small");
+        plan_launch(pl);
+        VLB_HIP(hipMemcpyAsync(out, pl.out.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        plan_sync(pl);
+    });
+}
+
+vlb_status vlb_bake_radiance(vlb_ctx* h, uint32_t scene, const vlb_bake_params* p, float* radiance,
+                             size_t radiance_len, int32_t* hit_prim, size_t hit_len)
+{
+    if (!h || !p) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        Scene& s = find_scene(*c, scene);
+        VLB_HIP(hipSetDevice(c->device));
+        Plan pl;
+        build_plan(*c, s, *p, pl);
+        const size_t nr = (size_t)pl.args.P * pl.args.R;
+        VLB_REQUIRE(!radiance || radiance_len >= nr * 3, VLB_ERR_INVALID, "radiance buffer too small");
+        VLB_REQUIRE(!hit_prim || hit_len >= nr, VLB_ERR_INVALID, "hit buffer too small");
+        DevBuf d_rad, d_hit;
+        if (radiance) {
+            d_rad = dev_alloc(*c, nr * 3 * sizeof(float));
+            pl.args.rad_out = d_rad.as<float>();
+        }
+        if (hit_prim) {
+            d_hit = dev_alloc(*c, nr * sizeof(int));
+            pl.args.hit_out = d_hit.as<int>();
+        }
+        launch_bake(pl.args, VLB_PREC_FP32, true, false, pl.grid, c->stream);
+        if (radiance)
+            VLB_HIP(hipMemcpyAsync(radiance, d_rad.p, nr * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                                   c->stream));
+        if (hit_prim)
+            VLB_HIP(hipMemcpyAsync(hit_prim, d_hit.p, nr * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        VLB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+// ---------------------------------------------------------- skybox / images
+static void ensure_wtab(Context& c, int N)
+{
+    if (c.wtab_n == N && c.wtab.p) return;
+    std::vector<float> w = solid_angle_table(N);
+    c.wtab = upload_vec(c, w);
+    c.wtab_n = N;
+}
+
+static int skybox_blocks_per_cube(int N, int batch)
+{
+    const int T = 6 * N * N;
+    // ~2048+ workgroups in flight, 256..8192 texels each
+    long long per = (long long)T * batch / 2048;
+    per = std::max(256LL, std::min(8192LL, per));
+    per = (per + 255) / 256 * 256;
+    return (int)((T + per - 1) / per);
+}
+
+vlb_status vlb_skybox_sh_d(vlb_ctx* h, const void* d_cubes, uint32_t face_size, uint32_t batch,
+                           uint32_t sh_order, void* d_out)
+{
+    if (!h || !d_cubes || !d_out) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_REQUIRE(face_size >= 1 && face_size <= 8192, VLB_ERR_INVALID, "face_size out of range");
+        VLB_REQUIRE(batch >= 1 && (uint64_t)batch * 6 * face_size * face_size < (1ull << 31),
+                    VLB_ERR_INVALID, "batch out of range");
+        VLB_REQUIRE(sh_order == 2 || sh_order == 3, VLB_ERR_INVALID, "sh_order must be 2 or 3");
+        VLB_HIP(hipSetDevice(c->device));
+        const int N = (int)face_size, K = sh_order == 2 ? 9 : 16;
+        ensure_wtab(*c, N);
+        const int bpc = skybox_blocks_per_cube(N, (int)batch);
+        const size_t pbytes = (size_t)batch * bpc * K * 3 * sizeof(float);
+        if (c->sky_partial.bytes < pbytes) c->sky_partial = dev_alloc(*c, pbytes);
+        launch_skybox_sh(static_cast<const float4*>(d_cubes), N, (int)batch, K, c->wtab.as<float>(),
+                         c->sky_partial.as<float>(), bpc, static_cast<float*>(d_out), c->stream,
+                         c->ev[0], c->ev[1], c->ev[2]);
+        c->sky_pending = true;
+    });
+}
+
+vlb_status vlb_skybox_last_ms(vlb_ctx* h, double* project_ms, double* merge_ms)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_HIP(hipStreamSynchronize(c->stream));
+        if (c->sky_pending) {
+            float a = 0.f, b = 0.f;
+            VLB_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+            VLB_HIP(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+            c->sky_project_ms = a;
+            c->sky_merge_ms = b;
+            c->sky_pending = false;
+        }
+        if (project_ms) *project_ms = c->sky_project_ms;
+        if (merge_ms) *merge_ms = c->sky_merge_ms;
+    });
+}
+
+vlb_status vlb_skybox_sh(vlb_ctx* h, const float* cubes, uint32_t face_size, uint32_t batch,
+                         uint32_t sh_order, float* out, size_t out_len)
+{
+    if (!h || !cubes || !out) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_REQUIRE(sh_order == 2 || sh_order == 3, VLB_ERR_INVALID, "sh_order must be 2 or 3");
+        VLB_REQUIRE(face_size >= 1 && face_size <= 8192 && batch >= 1, VLB_ERR_INVALID, "bad size");
+        const int K = sh_order == 2 ? 9 : 16;
+        const size_t n_out = (size_t)batch * K * 3;
+        VLB_REQUIRE(out_len >= n_out, VLB_ERR_INVALID, "output buffer too small");
+        VLB_HIP(hipSetDevice(c->device));
+        const size_t in_bytes = (size_t)batch * 6 * face_size * face_size * 16;
+        DevBuf d_in = dev_alloc(*c, in_bytes);
+        DevBuf d_out = dev_alloc(*c, n_out * sizeof(float));
+        VLB_HIP(hipMemcpyAsync(d_in.p, cubes, in_bytes, hipMemcpyHostToDevice, c->stream));
+        vlb_status s = vlb_skybox_sh_d(h, d_in.p, face_size, batch, sh_order, d_out.p);
+        if (s != VLB_OK) throw Error{s, c->last_error};
+        VLB_HIP(hipMemcpyAsync(out, d_out.p, n_out * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        VLB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+vlb_status vlb_project_equirect(vlb_ctx* h, const uint8_t* rgba8, uint32_t width, uint32_t height,
+                                int32_t variant, float* out16x3)
+{
+    if (!h || !rgba8 || !out16x3) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_REQUIRE(width >= 1 && height >= 1 && (uint64_t)width * height < (1ull << 30),
+                    VLB_ERR_INVALID, "bad image size");
+        VLB_REQUIRE(variant == VLB_EQUIRECT_SH_COMP || variant == VLB_EQUIRECT_SKYBOX_COMP,
+                    VLB_ERR_INVALID, "bad variant");
+        VLB_HIP(hipSetDevice(c->device));
+        const size_t n = (size_t)width * height;
+        const int nblocks = (int)std::max<size_t>(1, (n + 4095) / 4096);
+        DevBuf d_img = dev_alloc(*c, n * 4);
+        DevBuf d_part = dev_alloc(*c, (size_t)nblocks * 48 * sizeof(float));
+        DevBuf d_out = dev_alloc(*c, 48 * sizeof(double));
+        VLB_HIP(hipMemcpyAsync(d_img.p, rgba8, n * 4, hipMemcpyHostToDevice, c->stream));
+        launch_equirect_project(d_img.as<uint8_t>(), (int)width, (int)height, variant,
+                                d_out.as<double>(), d_part.as<float>(), nblocks, c->stream);
+        double o[48];
+        VLB_HIP(hipMemcpyAsync(o, d_out.p, sizeof o, hipMemcpyDeviceToHost, c->stream));
+        VLB_HIP(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < 48; ++i) out16x3[i] = (float)o[i];
+    });
+}
+
+vlb_status vlb_sh_reconstruct(vlb_ctx* h, const float* coeffs, uint32_t width, uint32_t height,
+                              uint8_t* out)
+{
+    if (!h || !coeffs || !out) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_REQUIRE(width >= 1 && height >= 1 && (uint64_t)width * height < (1ull << 30),
+                    VLB_ERR_INVALID, "bad image size");
+        VLB_HIP(hipSetDevice(c->device));
+        const size_t n = (size_t)width * height;
+        DevBuf d_c = dev_alloc(*c, 48 * sizeof(float));
+        DevBuf d_o = dev_alloc(*c, n * 4);
+        VLB_HIP(hipMemcpyAsync(d_c.p, coeffs, 48 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        launch_sh_reconstruct(d_c.as<float>(), (int)width, (int)height, d_o.as<uint8_t>(), c->stream);
+        VLB_HIP(hipMemcpyAsync(out, d_o.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+        VLB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+// ---------------------------------------------------------- device buffers
+vlb_status vlb_device_alloc(vlb_ctx* h, size_t bytes, void** dptr)
+{
+    if (!h || !dptr) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_HIP(hipSetDevice(c->device));
+        DevBuf b = dev_alloc(*c, bytes);
+        *dptr = b.p;
+        c->user_bufs[b.p] = std::move(b);
+    });
+}
+
+vlb_status vlb_device_free(vlb_ctx* h, void* dptr)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        auto it = c->user_bufs.find(dptr);
+        VLB_REQUIRE(it != c->user_bufs.end(), VLB_ERR_NOT_FOUND, "pointer not allocated by vlb_device_alloc");
+        VLB_HIP(hipStreamSynchronize(c->stream));
+        c->user_bufs.erase(it);
+    });
+}
+
+vlb_status vlb_upload(vlb_ctx* h, void* dptr, const void* src, size_t bytes)
+{
+    if (!h || !dptr || !src) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_HIP(hipMemcpyAsync(dptr, src, bytes, hipMemcpyHostToDevice, c->stream));
+        VLB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+vlb_status vlb_download(vlb_ctx* h, void* dst, const void* dptr, size_t bytes)
+{
+    if (!h || !dptr || !dst) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_HIP(hipMemcpyAsync(dst, dptr, bytes, hipMemcpyDeviceToHost, c->stream));
+        VLB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+vlb_status vlb_ctx_sync(vlb_ctx* h)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    return guard(c, [&] { VLB_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+}  // extern "C"

@@ -1,0 +1,492 @@
+// device_common.hpp -- device-side building blocks of the probe baker:
+// BVH node layout, watertight ray/triangle test, conservative slab test,
+// LDS-stack traversal (closest hit and shadow query), sky and shading.
+//
+// Every translation unit that includes this file is compiled with
+// -ffp-contract=off: the intersection, hit-point, normal and shadow-ray
+// arithmetic below is a fixed sequence of correctly rounded IEEE operations
+// (HIP's default f32 division and sqrt are correctly rounded), so hit ids and
+// visibility are bit-identical to a C restatement that does the same
+// operations in the same order.  Only the slab test uses explicit fmaf: it
+// decides which nodes are visited, never which triangle wins.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/vlb_bake.h"
+
+namespace vlb {
+
+// ---------------------------------------------------------------- layout
+// One BVH2 inner node = 64 bytes: the boxes of both children and their ids.
+// child >= 0: inner node index; child < 0: leaf, ~child = leaf-order
+// triangle index.  The root is node 0.
+struct __align__(16) BNode {
+    float4 a;  // lo0.x lo0.y lo0.z hi0.x
+    float4 b;  // hi0.y hi0.z lo1.x lo1.y
+    float4 c;  // lo1.z hi1.x hi1.y hi1.z
+    int4   d;  // child0 child1 - -
+};
+
+// Leaf-order triangle = 3 x float4: (v0, prim id bits) (v1, 0) (v2, 0).
+// Shading record per prim id = 3 x float4: (n0, albedo.r) (n1, .g) (n2, .b).
+
+struct SceneView {
+    const BNode*  nodes;
+    const float4* tris;
+    const float4* shade;
+    int           ntris;
+    float         maxabs;  // max |coordinate| of the scene's vertices
+};
+
+struct ShadeParams {
+    int32_t      mode;         // VLB_SHADE_*
+    int32_t      nlights;
+    float        shadow_bias;  // LAMBERT
+    float        t_max;        // shadow extent of directional lights
+    vlb_light    lights[VLB_MAX_LIGHTS];
+    vlb_sky      sky;
+    vlb_ref_shading ref;
+};
+
+constexpr int kLdsDepth = 32;  // stack entries kept in LDS per lane
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ float comp(const float3& v, int k)
+{
+    return k == 0 ? v.x : (k == 1 ? v.y : v.z);
+}
+__device__ __forceinline__ float dot3(const float3& a, const float3& b)
+{
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+// -------------------------------------------------------- watertight test
+// Woop, Benthin, Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1),
+// 2013: shear the triangle into ray space along the dominant axis kz, take
+// the 2D edge functions U, V, W (recomputed in double when one is zero), and
+// accept when they share a sign.
+struct WtRay {
+    float3 o;
+    int    kx, ky, kz;
+    float  Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ WtRay wt_setup(const float3& o, const float3& d)
+{
+    WtRay w;
+    w.o = o;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+    int kx = kz + 1;
+    if (kx == 3) kx = 0;
+    int ky = kx + 1;
+    if (ky == 3) ky = 0;
+    float dkz = comp(d, kz);
+    if (dkz < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    w.kx = kx;
+    w.ky = ky;
+    w.kz = kz;
+    w.Sx = comp(d, kx) / dkz;
+    w.Sy = comp(d, ky) / dkz;
+    w.Sz = 1.0f / dkz;
+    return w;
+}
+
+__device__ __forceinline__ bool wt_hit(const WtRay& w, const float3& v0, const float3& v1,
+                                       const float3& v2, float tmin, float tmax, float& t_out,
+                                       float& u_out, float& v_out)
+{
+    const float3 A = make_float3(v0.x - w.o.x, v0.y - w.o.y, v0.z - w.o.z);
+    const float3 B = make_float3(v1.x - w.o.x, v1.y - w.o.y, v1.z - w.o.z);
+    const float3 C = make_float3(v2.x - w.o.x, v2.y - w.o.y, v2.z - w.o.z);
+    const float Akz = comp(A, w.kz), Bkz = comp(B, w.kz), Ckz = comp(C, w.kz);
+    const float Ax = comp(A, w.kx) - w.Sx * Akz;
+    const float Ay = comp(A, w.ky) - w.Sy * Akz;
+    const float Bx = comp(B, w.kx) - w.Sx * Bkz;
+    const float By = comp(B, w.ky) - w.Sy * Bkz;
+    const float Cx = comp(C, w.kx) - w.Sx * Ckz;
+    const float Cy = comp(C, w.ky) - w.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        double CxBy = (double)Cx * (double)By;
+        double CyBx = (double)Cy * (double)Bx;
+        U = (float)(CxBy - CyBx);
+        double AxCy = (double)Ax * (double)Cy;
+        double AyCx = (double)Ay * (double)Cx;
+        V = (float)(AxCy - AyCx);
+        double BxAy = (double)Bx * (double)Ay;
+        double ByAx = (double)By * (double)Ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = w.Sz * Akz;
+    const float Bz = w.Sz * Bkz;
+    const float Cz = w.Sz * Ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    const float t = T / det;
+    if (!(t >= tmin && t <= tmax)) return false;
+    t_out = t;
+    u_out = V / det;
+    v_out = W / det;
+    return true;
+}
+
+// -------------------------------------------------------------- slab test
+// Conservative: widened by 4e-6 relative plus an absolute margin of 64 ulps
+// of the largest coordinate in play, so a box holding an accepted triangle
+// is never culled.  Margins cost node visits, never change a result.
+struct SlabRay {
+    float3 inv;
+    float3 oinv;  // o * inv
+    float  apad;
+};
+
+__device__ __forceinline__ SlabRay slab_setup(const float3& o, const float3& d, float maxabs)
+{
+    SlabRay s;
+    float dx = d.x == 0.0f ? copysignf(1e-20f, d.x) : d.x;
+    float dy = d.y == 0.0f ? copysignf(1e-20f, d.y) : d.y;
+    float dz = d.z == 0.0f ? copysignf(1e-20f, d.z) : d.z;
+    s.inv = make_float3(1.0f / dx, 1.0f / dy, 1.0f / dz);
+    s.oinv = make_float3(o.x * s.inv.x, o.y * s.inv.y, o.z * s.inv.z);
+    float m = fmaxf(maxabs, fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z))));
+    s.apad = m * (64.0f * 1.1920929e-7f);
+    return s;
+}
+
+// returns the entry distance (for near-first ordering) or +inf on a miss
+__device__ __forceinline__ float slab(const SlabRay& s, float lox, float loy, float loz,
+                                      float hix, float hiy, float hiz, float tmin, float tbest)
+{
+    float t0x = fmaf(lox, s.inv.x, -s.oinv.x), t1x = fmaf(hix, s.inv.x, -s.oinv.x);
+    float t0y = fmaf(loy, s.inv.y, -s.oinv.y), t1y = fmaf(hiy, s.inv.y, -s.oinv.y);
+    float t0z = fmaf(loz, s.inv.z, -s.oinv.z), t1z = fmaf(hiz, s.inv.z, -s.oinv.z);
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    float hi = fminf(tf, tbest);
+    hi = hi + fabsf(hi) * 4e-6f + s.apad;
+    float lo = tn - fabsf(tn) * 4e-6f - s.apad;
+    return (lo <= hi && hi >= tmin) ? lo : __builtin_huge_valf();
+}
+
+// ------------------------------------------------------------- traversal
+// Stack: entries [0, kLdsDepth) in LDS at lds[i * 256] (lane-interleaved,
+// conflict-free), deeper entries in a global spill area at
+// spill[(i - kLdsDepth) * spill_stride].
+struct Stack {
+    uint32_t* lds;
+    uint32_t* spill;
+    uint32_t  spill_stride;
+};
+
+__device__ __forceinline__ void push(const Stack& st, int& sp, uint32_t v)
+{
+    if (sp < kLdsDepth) st.lds[sp * 256] = v;
+    else st.spill[(size_t)(sp - kLdsDepth) * st.spill_stride] = v;
+    ++sp;
+}
+__device__ __forceinline__ uint32_t pop(const Stack& st, int& sp)
+{
+    --sp;
+    if (sp < kLdsDepth) return st.lds[sp * 256];
+    return st.spill[(size_t)(sp - kLdsDepth) * st.spill_stride];
+}
+
+struct TraceCount {
+    uint32_t nodes = 0;
+    uint32_t tris = 0;
+};
+
+__device__ __forceinline__ void load_tri(const float4* tris, int leaf, float3& v0, float3& v1,
+                                         float3& v2, int& prim)
+{
+    const float4 a = tris[3 * leaf + 0];
+    const float4 b = tris[3 * leaf + 1];
+    const float4 c = tris[3 * leaf + 2];
+    v0 = make_float3(a.x, a.y, a.z);
+    v1 = make_float3(b.x, b.y, b.z);
+    v2 = make_float3(c.x, c.y, c.z);
+    prim = __float_as_int(a.w);
+}
+
+// Closest hit: lexicographic minimum of (t, prim id) over accepted
+// triangles in [tmin, tmax].  Returns the prim id or -1.
+template <bool COUNT>
+__device__ int trace_closest(const SceneView& sc, const float3& o, const float3& d, float tmin,
+                             float tmax, float& t_out, float& u_out, float& v_out,
+                             const Stack& st, TraceCount& cnt)
+{
+    if (sc.ntris <= 0) return -1;
+    const WtRay w = wt_setup(o, d);
+    const SlabRay s = slab_setup(o, d, sc.maxabs);
+    float tbest = tmax, bu = 0.0f, bv = 0.0f;
+    int best = -1;
+    int sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        const BNode n = sc.nodes[node];
+        if (COUNT) ++cnt.nodes;
+        float e0 = slab(s, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, tmin, tbest);
+        float e1 = slab(s, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, tmin, tbest);
+        int c0 = n.d.x, c1 = n.d.y;
+        bool h0 = e0 != __builtin_huge_valf();
+        bool h1 = e1 != __builtin_huge_valf();
+        if (h0 && c0 < 0) {
+            float3 v0, v1, v2;
+            int prim;
+            load_tri(sc.tris, ~c0, v0, v1, v2, prim);
+            if (COUNT) ++cnt.tris;
+            float t, u, v;
+            if (wt_hit(w, v0, v1, v2, tmin, tbest, t, u, v) &&
+                (best < 0 || t < tbest || prim < best)) {
+                tbest = t;
+                best = prim;
+                bu = u;
+                bv = v;
+            }
+            h0 = false;
+        }
+        if (h1 && c1 < 0) {
+            float3 v0, v1, v2;
+            int prim;
+            load_tri(sc.tris, ~c1, v0, v1, v2, prim);
+            if (COUNT) ++cnt.tris;
+            float t, u, v;
+            if (wt_hit(w, v0, v1, v2, tmin, tbest, t, u, v) &&
+                (best < 0 || t < tbest || prim < best)) {
+                tbest = t;
+                best = prim;
+                bu = u;
+                bv = v;
+            }
+            h1 = false;
+        }
+        if (h0 && h1) {
+            const bool first0 = e0 <= e1;
+            push(st, sp, (uint32_t)(first0 ? c1 : c0));
+            node = (uint32_t)(first0 ? c0 : c1);
+        } else if (h0) {
+            node = (uint32_t)c0;
+        } else if (h1) {
+            node = (uint32_t)c1;
+        } else {
+            if (sp == 0) break;
+            node = pop(st, sp);
+        }
+    }
+    if (best >= 0) {
+        t_out = tbest;
+        u_out = bu;
+        v_out = bv;
+    }
+    return best;
+}
+
+// Shadow query: true if any triangle is accepted in [tmin, tmax].
+template <bool COUNT>
+__device__ bool trace_any(const SceneView& sc, const float3& o, const float3& d, float tmin,
+                          float tmax, const Stack& st, TraceCount& cnt)
+{
+    if (sc.ntris <= 0) return false;
+    const WtRay w = wt_setup(o, d);
+    const SlabRay s = slab_setup(o, d, sc.maxabs);
+    int sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        const BNode n = sc.nodes[node];
+        if (COUNT) ++cnt.nodes;
+        float e0 = slab(s, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, tmin, tmax);
+        float e1 = slab(s, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, tmin, tmax);
+        int c0 = n.d.x, c1 = n.d.y;
+        bool h0 = e0 != __builtin_huge_valf();
+        bool h1 = e1 != __builtin_huge_valf();
+        if (h0 && c0 < 0) {
+            float3 v0, v1, v2;
+            int prim;
+            load_tri(sc.tris, ~c0, v0, v1, v2, prim);
+            if (COUNT) ++cnt.tris;
+            float t, u, v;
+            if (wt_hit(w, v0, v1, v2, tmin, tmax, t, u, v)) return true;
+            h0 = false;
+        }
+        if (h1 && c1 < 0) {
+            float3 v0, v1, v2;
+            int prim;
+            load_tri(sc.tris, ~c1, v0, v1, v2, prim);
+            if (COUNT) ++cnt.tris;
+            float t, u, v;
+            if (wt_hit(w, v0, v1, v2, tmin, tmax, t, u, v)) return true;
+            h1 = false;
+        }
+        if (h0 && h1) {
+            const bool first0 = e0 <= e1;
+            push(st, sp, (uint32_t)(first0 ? c1 : c0));
+            node = (uint32_t)(first0 ? c0 : c1);
+        } else if (h0) {
+            node = (uint32_t)c0;
+        } else if (h1) {
+            node = (uint32_t)c1;
+        } else {
+            if (sp == 0) break;
+            node = pop(st, sp);
+        }
+    }
+    return false;
+}
+
+// ------------------------------------------------------------------- sky
+__device__ __forceinline__ float3 sky_radiance(const vlb_sky& sky, const float3& d)
+{
+    float g = d.y > 0.0f ? d.y : 0.0f;
+    float sd = d.x * sky.sun_dir[0] + d.y * sky.sun_dir[1] + d.z * sky.sun_dir[2];
+    float sun = expf(sky.sun_sharpness * (sd - 1.0f));
+    float3 r;
+    r.x = sky.horizon_rgb[0] + (sky.zenith_rgb[0] - sky.horizon_rgb[0]) * g + sky.sun_rgb[0] * sun;
+    r.y = sky.horizon_rgb[1] + (sky.zenith_rgb[1] - sky.horizon_rgb[1]) * g + sky.sun_rgb[1] * sun;
+    r.z = sky.horizon_rgb[2] + (sky.zenith_rgb[2] - sky.horizon_rgb[2]) * g + sky.sun_rgb[2] * sun;
+    return r;
+}
+
+// env_map.rchit:27-34
+__device__ __forceinline__ float srgb(float x)
+{
+    if (x < 0.0031308f) return x * 12.92f;
+    return 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
+}
+// RGBA8 storage-image store followed by a load
+__device__ __forceinline__ float unorm8(float x)
+{
+    float c = (x > 0.0f) ? (x < 1.0f ? x : 1.0f) : 0.0f;
+    return rintf(c * 255.0f) / 255.0f;
+}
+
+// ---------------------------------------------------------------- shading
+// Radiance carried by one primary ray.  `nshadow` counts traced shadow rays.
+template <bool COUNT>
+__device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const float3& o,
+                            const float3& d, float tmin, float tmax, const Stack& st,
+                            TraceCount& cnt, uint32_t& nshadow, int& hit_prim)
+{
+    float t = 0.0f, u = 0.0f, v = 0.0f;
+    const int prim = trace_closest<COUNT>(sc, o, d, tmin, tmax, t, u, v, st, cnt);
+    hit_prim = prim;
+    const bool ref = sp.mode == VLB_SHADE_REFERENCE;
+    if (prim < 0) {
+        float3 s = sky_radiance(sp.sky, d);
+        if (ref) {
+            if (sp.ref.flags & VLB_REF_SRGB) {
+                s.x = srgb(s.x);
+                s.y = srgb(s.y);
+                s.z = srgb(s.z);
+            }
+            if (sp.ref.flags & VLB_REF_UNORM8) {
+                s.x = unorm8(s.x);
+                s.y = unorm8(s.y);
+                s.z = unorm8(s.z);
+            }
+        }
+        return s;
+    }
+    // env_map.rchit:63-68
+    const float4 s0 = sc.shade[3 * prim + 0];
+    const float4 s1 = sc.shade[3 * prim + 1];
+    const float4 s2 = sc.shade[3 * prim + 2];
+    const float b0 = 1.0f - u - v;
+    const float3 P = make_float3(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+    float3 m;
+    m.x = s0.x * b0 + s1.x * u + s2.x * v;
+    m.y = s0.y * b0 + s1.y * u + s2.y * v;
+    m.z = s0.z * b0 + s1.z * u + s2.z * v;
+    const float ml = sqrtf(dot3(m, m));
+    float3 N = ml > 0.0f ? make_float3(m.x / ml, m.y / ml, m.z / ml) : make_float3(0.f, 0.f, 0.f);
+    const float3 alb = make_float3(s0.w, s1.w, s2.w);
+    if (ref) {
+        // env_map.rchit:73-101, main.rchit:95-116
+        const vlb_ref_shading& r = sp.ref;
+        const float3 L = make_float3(r.light_position[0] - P.x, r.light_position[1] - P.y,
+                                     r.light_position[2] - P.z);
+        const float Llen = sqrtf(dot3(L, L));
+        const float3 Ln = Llen > 0.0f ? make_float3(L.x / Llen, L.y / Llen, L.z / Llen)
+                                      : make_float3(0.f, 0.f, 0.f);
+        const float sdn = dot3(Ln, N);
+        const float sDotN = sdn > 0.0f ? sdn : 0.0f;
+        const float3 org = make_float3(P.x + r.shadow_bias * N.x, P.y + r.shadow_bias * N.y,
+                                       P.z + r.shadow_bias * N.z);
+        bool in_shadow = true;
+        if (sDotN != 0.0f) {
+            ++nshadow;
+            in_shadow = trace_any<COUNT>(sc, org, Ln, 0.0f, Llen, st, cnt);
+        }
+        float diffuse = 0.0f, spec = 0.0f;
+        if (!in_shadow) {
+            diffuse = r.c_diffuse * sDotN;
+            const float dnl = dot3(N, Ln);
+            const float3 R = make_float3(Ln.x - 2.0f * dnl * N.x, Ln.y - 2.0f * dnl * N.y,
+                                         Ln.z - 2.0f * dnl * N.z);
+            const float rd = dot3(R, d);
+            spec = r.c_specular * powf(rd > 0.0f ? rd : 0.0f, r.glossiness);
+        }
+        const bool zero = alb.x == 0.0f && alb.y == 0.0f && alb.z == 0.0f;
+        float3 c;
+        c.x = (zero ? 1.0f : alb.x) * (r.ambient + diffuse + spec);
+        c.y = (zero ? 1.0f : alb.y) * (r.ambient + diffuse + spec);
+        c.z = (zero ? 1.0f : alb.z) * (r.ambient + diffuse + spec);
+        if (r.flags & VLB_REF_SRGB) {
+            c.x = srgb(c.x);
+            c.y = srgb(c.y);
+            c.z = srgb(c.z);
+        }
+        if (r.flags & VLB_REF_UNORM8) {
+            c.x = unorm8(c.x);
+            c.y = unorm8(c.y);
+            c.z = unorm8(c.z);
+        }
+        return c;
+    }
+    // LAMBERT, two-sided
+    if (dot3(N, d) > 0.0f) N = make_float3(-N.x, -N.y, -N.z);
+    float3 acc = make_float3(0.f, 0.f, 0.f);
+    for (int j = 0; j < sp.nlights; ++j) {
+        const vlb_light& Lj = sp.lights[j];
+        float3 Ld;
+        float dist;
+        float3 E;
+        if (Lj.type == VLB_LIGHT_POINT) {
+            const float3 Lv = make_float3(Lj.vec[0] - P.x, Lj.vec[1] - P.y, Lj.vec[2] - P.z);
+            dist = sqrtf(dot3(Lv, Lv));
+            Ld = dist > 0.0f ? make_float3(Lv.x / dist, Lv.y / dist, Lv.z / dist)
+                             : make_float3(0.f, 0.f, 0.f);
+            const float d2 = dist * dist;
+            E = make_float3(Lj.rgb[0] / d2, Lj.rgb[1] / d2, Lj.rgb[2] / d2);
+        } else {
+            Ld = make_float3(-Lj.vec[0], -Lj.vec[1], -Lj.vec[2]);
+            dist = sp.t_max;
+            E = make_float3(Lj.rgb[0], Lj.rgb[1], Lj.rgb[2]);
+        }
+        const float cosNL = dot3(N, Ld);
+        if (cosNL > 0.0f) {
+            const float3 org = make_float3(P.x + sp.shadow_bias * N.x, P.y + sp.shadow_bias * N.y,
+                                           P.z + sp.shadow_bias * N.z);
+            ++nshadow;
+            if (!trace_any<COUNT>(sc, org, Ld, 0.0f, dist, st, cnt)) {
+                acc.x = acc.x + E.x * cosNL;
+                acc.y = acc.y + E.y * cosNL;
+                acc.z = acc.z + E.z * cosNL;
+            }
+        }
+    }
+    const float kInvPi = 0.318309886183790671538f;
+    return make_float3(alb.x * acc.x * kInvPi, alb.y * acc.y * kInvPi, alb.z * acc.z * kInvPi);
+}
+
+}  // namespace vlb

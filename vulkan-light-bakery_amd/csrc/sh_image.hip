@@ -1,0 +1,296 @@
+// sh_image.hip -- SH projection of skies and images.
+//
+//  k_skybox    : the build's form of skybox_sh.comp (shaders/skybox_sh.comp:
+//                23-42, host src/skybox_manager.cpp:107-130): batched
+//                6 x N x N RGBA32F cubemaps -> K coefs x RGB, exact texel
+//                solid angles.  HBM-bound: 16 B per texel streamed once.
+//  k_equirect  : shaders/sh.comp:23-42 and skybox_sh.comp:23-42 over an RGBA8
+//                equirect image (intended sum: no race, fresh zero).
+//  k_sh_recon  : shaders/sh_sum.comp:35-52 (1250 * sum_k c_k Y_k -> RGBA8).
+//
+// Reductions: per-thread FP32 accumulators, a wave butterfly (xor shuffles),
+// an LDS combine across the 4 waves, one partial per workgroup, then an FP64
+// merge kernel that adds the partials in a fixed order.  No float atomics,
+// so results are deterministic (sh.comp's shared "+=" is a data race).
+#include "vlb_internal.hpp"
+
+namespace vlb {
+namespace {
+
+// Real SH with exact normalisation, l <= 3, SH frame argument (x, y, z).
+template <int K>
+__device__ __forceinline__ void sh_exact(float x, float y, float z, float* Y)
+{
+    Y[0] = 0.28209479177387814f;
+    Y[1] = -0.48860251190291992f * y;
+    Y[2] = 0.48860251190291992f * z;
+    Y[3] = -0.48860251190291992f * x;
+    Y[4] = 1.0925484305920792f * x * y;
+    Y[5] = -1.0925484305920792f * y * z;
+    Y[6] = 0.31539156525252005f * (2.0f * z * z - x * x - y * y);
+    Y[7] = -1.0925484305920792f * x * z;
+    Y[8] = 0.54627421529603959f * (x * x - y * y);
+    if (K > 9) {
+        Y[9] = -0.59004358992664352f * y * (3.0f * x * x - y * y);
+        Y[10] = 2.8906114426405538f * x * y * z;
+        Y[11] = -0.45704579946446572f * y * (4.0f * z * z - x * x - y * y);
+        Y[12] = 0.37317633259011540f * z * (2.0f * z * z - 3.0f * x * x - 3.0f * y * y);
+        Y[13] = -0.45704579946446572f * x * (4.0f * z * z - x * x - y * y);
+        Y[14] = 1.4453057213202769f * z * (x * x - y * y);
+        Y[15] = -0.59004358992664352f * x * (x * x - 3.0f * y * y);
+    }
+}
+
+// The literal 6-figure constants of shaders/sh_common.h:26-116, l <= 3, in the
+// GLSL operation order.
+__device__ __forceinline__ void sh_literal16(float x, float y, float z, float* v)
+{
+    v[0] = 0.282095f;
+    v[1] = -0.488603f * y;
+    v[2] = 0.488603f * z;
+    v[3] = -0.488603f * x;
+    v[4] = 1.092548f * x * y;
+    v[5] = -1.092548f * y * z;
+    v[6] = 0.315392f * (-x * x - y * y + 2.0f * z * z);
+    v[7] = -1.092548f * x * z;
+    v[8] = 0.546274f * (x * x - y * y);
+    v[9] = -0.590044f * y * (3.0f * x * x - y * y);
+    v[10] = 2.890611f * x * y * z;
+    v[11] = -0.457046f * y * (4.0f * z * z - x * x - y * y);
+    v[12] = 0.373176f * z * (2.0f * z * z - 3.0f * x * x - 3.0f * y * y);
+    v[13] = -0.457046f * x * (4.0f * z * z - x * x - y * y);
+    v[14] = 1.445306f * z * (x * x - y * y);
+    v[15] = -0.590044f * x * (x * x - 3.0f * y * y);
+}
+
+// Sum NV values over the workgroup; thread 0's acc holds the result.
+template <int NV>
+__device__ __forceinline__ void block_reduce(float* acc, float* lds)
+{
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        float v = acc[k];
+        v += __shfl_xor(v, 32);
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 8);
+        v += __shfl_xor(v, 4);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 1);
+        acc[k] = v;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < NV; ++k) lds[wave * NV + k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; ++k) acc[k] = ((lds[k] + lds[NV + k]) + lds[2 * NV + k]) + lds[3 * NV + k];
+}
+
+// Vulkan cube-map face orientation (faces +X,-X,+Y,-Y,+Z,-Z; s = u, t = v).
+__device__ __forceinline__ float3 cube_dir(int f, float u, float v)
+{
+    float3 d;
+    switch (f) {
+    case 0: d = make_float3(1.0f, -v, -u); break;
+    case 1: d = make_float3(-1.0f, -v, u); break;
+    case 2: d = make_float3(u, 1.0f, v); break;
+    case 3: d = make_float3(u, -1.0f, -v); break;
+    case 4: d = make_float3(u, -v, 1.0f); break;
+    default: d = make_float3(-u, -v, -1.0f); break;
+    }
+    const float inv = 1.0f / sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    return make_float3(d.x * inv, d.y * inv, d.z * inv);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_skybox(const float4* __restrict__ cubes, int N, int logN,
+                                                int per_block, int blocks_per_cube,
+                                                const float* __restrict__ wtab,
+                                                float* __restrict__ partial)
+{
+    __shared__ float lds[4 * K * 3];
+    const int cube = blockIdx.x / blocks_per_cube;
+    const int blk = blockIdx.x - cube * blocks_per_cube;
+    const int NN = N * N;
+    const int T = 6 * NN;
+    const int start = blk * per_block;
+    const int end = min(T, start + per_block);
+    const float4* src = cubes + (size_t)cube * T;
+    const float s2 = 2.0f / (float)N, s1 = 1.0f / (float)N - 1.0f;
+    float acc[K * 3];
+#pragma unroll
+    for (int k = 0; k < K * 3; ++k) acc[k] = 0.0f;
+    for (int t = start + (int)threadIdx.x; t < end; t += 256) {
+        typedef float nf4 __attribute__((ext_vector_type(4)));
+        const nf4 px = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(src + t));
+        int f, rem, j, i;
+        if (logN >= 0) {
+            f = t >> (2 * logN);
+            rem = t & (NN - 1);
+            j = rem >> logN;
+            i = rem & (N - 1);
+        } else {
+            f = t / NN;
+            rem = t - f * NN;
+            j = rem / N;
+            i = rem - j * N;
+        }
+        const float u = fmaf((float)i, s2, s1);
+        const float v = fmaf((float)j, s2, s1);
+        const float3 d = cube_dir(f, u, v);
+        const float w = wtab[rem];
+        const float r = px[0] * w, g = px[1] * w, b = px[2] * w;
+        float Y[16];
+        sh_exact<K>(d.x, d.z, d.y, Y);  // SH frame: world (x, z, y)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            acc[3 * k + 0] = fmaf(Y[k], r, acc[3 * k + 0]);
+            acc[3 * k + 1] = fmaf(Y[k], g, acc[3 * k + 1]);
+            acc[3 * k + 2] = fmaf(Y[k], b, acc[3 * k + 2]);
+        }
+    }
+    block_reduce<K * 3>(acc, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < K * 3; ++k) partial[(size_t)blockIdx.x * K * 3 + k] = acc[k];
+}
+
+// FP64 merge of per-workgroup partials: out[g][v] = sum_b partial[g][b][v].
+__global__ __launch_bounds__(256) void k_merge_partials(const float* __restrict__ partial,
+                                                        int groups, int per_group, int nv,
+                                                        float* __restrict__ out,
+                                                        double* __restrict__ out64)
+{
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= groups * nv) return;
+    const int gi = idx / nv, v = idx - gi * nv;
+    double s = 0.0;
+    for (int b = 0; b < per_group; ++b) s += (double)partial[((size_t)gi * per_group + b) * nv + v];
+    if (out) out[idx] = (float)s;
+    if (out64) out64[idx] = s;
+}
+
+// sh.comp (variant 0) / skybox_sh.comp (variant 1) over an RGBA8 image.
+__global__ __launch_bounds__(256) void k_equirect(const uint8_t* __restrict__ img, int W, int H,
+                                                  int variant, int per_block,
+                                                  float* __restrict__ partial)
+{
+    __shared__ float lds[4 * 48];
+    const float PI = 3.1415926538f;  // sh_common.h:1
+    const int n = W * H;
+    const int start = blockIdx.x * per_block;
+    const int end = min(n, start + per_block);
+    const float area = (2.0f * PI / (float)W) * (PI / (float)H);
+    float acc[48];
+#pragma unroll
+    for (int k = 0; k < 48; ++k) acc[k] = 0.0f;
+    for (int p = start + (int)threadIdx.x; p < end; p += 256) {
+        const int y = p / W, x = p - y * W;
+        float phi = 2.f * PI * ((float)x + 0.5f) / (float)W;
+        if (variant == VLB_EQUIRECT_SKYBOX_COMP) phi -= PI / 2.0f;
+        const float theta = PI * ((float)y + 0.5f) / (float)H;
+        float st, ct, sp, cp;
+        sincosf(theta, &st, &ct);
+        sincosf(phi, &sp, &cp);
+        float dx = st * cp, dy = st * sp, dz = ct;
+        const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+        dx *= inv;
+        dy *= inv;
+        dz *= inv;
+        const uchar4 px = reinterpret_cast<const uchar4*>(img)[p];
+        const float w = area * st;
+        const float r = (float)px.x / 255.0f * w, g = (float)px.y / 255.0f * w,
+                    b = (float)px.z / 255.0f * w;
+        float Y[16];
+        if (variant == VLB_EQUIRECT_SKYBOX_COMP) sh_literal16(dx, dz, dy, Y);  // SH(dir.xzy)
+        else sh_literal16(dx, dy, dz, Y);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            acc[3 * k + 0] = fmaf(Y[k], r, acc[3 * k + 0]);
+            acc[3 * k + 1] = fmaf(Y[k], g, acc[3 * k + 1]);
+            acc[3 * k + 2] = fmaf(Y[k], b, acc[3 * k + 2]);
+        }
+    }
+    block_reduce<48>(acc, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 48; ++k) partial[(size_t)blockIdx.x * 48 + k] = acc[k];
+}
+
+// sh_sum.comp:35-52 in float, operation order of the GLSL.
+__global__ __launch_bounds__(256) void k_sh_recon(const float* __restrict__ coeffs, int W, int H,
+                                                  uint8_t* __restrict__ out)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= W * H) return;
+    const float PI = 3.1415926538f;
+    const int y = p / W, x = p - y * W;
+    const float phi = 2.f * PI * ((float)x + 0.5f) / (float)W;
+    const float theta = PI * ((float)y + 0.5f) / (float)H;
+    const float r = sinf(theta);
+    const float a = r * cosf(phi), b = r * sinf(phi), c = cosf(theta);
+    const float l = sqrtf(a * a + b * b + c * c);
+    float v[16];
+    sh_literal16(a / l, b / l, c / l, v);
+    float col[3] = {0.f, 0.f, 0.f};
+    for (int k = 0; k < 16; ++k)
+        for (int ch = 0; ch < 3; ++ch) col[ch] = col[ch] + coeffs[3 * k + ch] * v[k];
+    uchar4 o;
+    float s[3];
+    for (int ch = 0; ch < 3; ++ch) {
+        float t = 1250.0f * col[ch];
+        float cl = (t > 0.0f) ? (t < 1.0f ? t : 1.0f) : 0.0f;
+        s[ch] = rintf(cl * 255.0f);
+    }
+    o.x = (uint8_t)s[0];
+    o.y = (uint8_t)s[1];
+    o.z = (uint8_t)s[2];
+    o.w = 255;
+    reinterpret_cast<uchar4*>(out)[p] = o;
+}
+
+}  // namespace
+
+void launch_skybox_sh(const float4* cubes, int N, int batch, int K, const float* wtab,
+                      float* partial, int blocks_per_cube, float* out, hipStream_t st,
+                      hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
+{
+    const int T = 6 * N * N;
+    const int per_block = (T + blocks_per_cube - 1) / blocks_per_cube;
+    int logN = -1;
+    if ((N & (N - 1)) == 0) {
+        logN = 0;
+        while ((1 << logN) < N) ++logN;
+    }
+    if (e0) VLB_HIP(hipEventRecord(e0, st));
+    const int grid = batch * blocks_per_cube;
+    if (K == 16)
+        k_skybox<16><<<grid, 256, 0, st>>>(cubes, N, logN, per_block, blocks_per_cube, wtab, partial);
+    else
+        k_skybox<9><<<grid, 256, 0, st>>>(cubes, N, logN, per_block, blocks_per_cube, wtab, partial);
+    VLB_HIP(hipGetLastError());
+    if (e1) VLB_HIP(hipEventRecord(e1, st));
+    const int nv = K * 3;
+    k_merge_partials<<<(batch * nv + 255) / 256, 256, 0, st>>>(partial, batch, blocks_per_cube, nv,
+                                                                out, nullptr);
+    VLB_HIP(hipGetLastError());
+    if (e2) VLB_HIP(hipEventRecord(e2, st));
+}
+
+void launch_equirect_project(const uint8_t* img, int W, int H, int variant, double* out,
+                             float* partial, int nblocks, hipStream_t st)
+{
+    const int n = W * H;
+    const int per_block = (n + nblocks - 1) / nblocks;
+    k_equirect<<<nblocks, 256, 0, st>>>(img, W, H, variant, per_block, partial);
+    VLB_HIP(hipGetLastError());
+    k_merge_partials<<<1, 256, 0, st>>>(partial, 1, nblocks, 48, nullptr, out);
+    VLB_HIP(hipGetLastError());
+}
+
+void launch_sh_reconstruct(const float* coeffs, int W, int H, uint8_t* out, hipStream_t st)
+{
+    const int n = W * H;
+    k_sh_recon<<<(n + 255) / 256, 256, 0, st>>>(coeffs, W, H, out);
+    VLB_HIP(hipGetLastError());
+}
+
+}  // namespace vlb

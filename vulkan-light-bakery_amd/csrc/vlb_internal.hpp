@@ -1,0 +1,211 @@
+// vlb_internal.hpp -- host-side objects behind the C ABI: context, scene,
+// bake plan, and the launch functions each .hip translation unit exports.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vlb_bake.h"
+#include "device_common.hpp"
+
+namespace vlb {
+
+// ------------------------------------------------------------------ errors
+struct Error {
+    vlb_status code;
+    std::string msg;
+};
+
+#define VLB_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw ::vlb::Error{e_ == hipErrorOutOfMemory ? VLB_ERR_OOM : VLB_ERR_DEVICE, \
+                               std::string(#call) + ": " + hipGetErrorString(e_)};     \
+    } while (0)
+
+#define VLB_REQUIRE(cond, code, text)                           \
+    do {                                                        \
+        if (!(cond)) throw ::vlb::Error{(code), (text)};        \
+    } while (0)
+
+// --------------------------------------------------------- device buffers
+// Device allocation owned by RAII; `acct` is the owning context's byte
+// counter (HBM budget admission).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    std::atomic<uint64_t>* acct = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), acct(o.acct)
+    {
+        o.p = nullptr;
+        o.bytes = 0;
+        o.acct = nullptr;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            acct = o.acct;
+            o.p = nullptr;
+            o.bytes = 0;
+            o.acct = nullptr;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) {
+            (void)hipFree(p);
+            if (acct) acct->fetch_sub(bytes);
+        }
+        p = nullptr;
+        bytes = 0;
+        acct = nullptr;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Context;
+DevBuf dev_alloc(Context& ctx, size_t bytes);
+
+// ------------------------------------------------------------------ scene
+struct Scene {
+    uint32_t id = 0;
+    int ntris = 0;
+    int nnodes = 0;
+    DevBuf nodes, tris, shade;
+    float maxabs = 0.0f;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    uint32_t depth = 0;
+    double build_ms = 0.0;
+    size_t bytes() const { return nodes.bytes + tris.bytes + shade.bytes; }
+    SceneView view() const
+    {
+        SceneView v;
+        v.nodes = nodes.as<BNode>();
+        v.tris = tris.as<float4>();
+        v.shade = shade.as<float4>();
+        v.ntris = ntris;
+        v.maxabs = maxabs;
+        return v;
+    }
+};
+
+// ---------------------------------------------------------------- context
+struct Context {
+    int device = 0;
+    int num_cus = 0;
+    std::string name;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;                  // bytes
+    std::atomic<uint64_t> in_use{0};      // bytes allocated through dev_alloc
+    std::string last_error;
+    std::map<uint32_t, std::unique_ptr<Scene>> scenes;
+    uint32_t next_scene = 1;
+    // skybox state: events, cached solid-angle table and partial buffer
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    int wtab_n = 0;
+    DevBuf wtab;
+    DevBuf sky_partial;
+    double sky_project_ms = 0.0, sky_merge_ms = 0.0;
+    bool sky_pending = false;
+    // user allocations (vlb_device_alloc)
+    std::map<void*, DevBuf> user_bufs;
+    std::recursive_mutex mtx;
+};
+
+// ------------------------------------------------------------ bake plan
+struct BakeArgs {
+    SceneView sc;
+    ShadeParams sp;
+    const float4* probes;     // Ppad, w unused
+    int P, Ppad;
+    const float4* dirs;       // Rpad: xyz, w = 1 valid / 0 padding
+    const void* ytab;         // [Rpad/8][16][8] fp32 | bf16 | fp16
+    int R, Rpad, RS, nslices, npg, ntiles;
+    float tmin, tmax;
+    float2* partials;         // [nslices][Ppad][3][16] (sum, compensation)
+    uint32_t* spill;
+    uint32_t spill_stride;    // grid * 256
+    // record / count mode
+    float* rad_out;           // [P][R][3] canonical order (probe_perm, dir_perm)
+    int* hit_out;             // [P][R]
+    const int* probe_perm;    // sorted probe -> original index (-1 padding)
+    const int* dir_perm;      // sorted ray -> canonical index
+    unsigned long long* counters;  // node visits, tri tests, shadow rays
+};
+
+struct MergeArgs {
+    const float2* partials;
+    int nslices, P, Ppad, K;
+    const int* probe_perm;
+    double scale;
+    float* out;               // [P][K][3]
+    double* out64;            // [P][K][3]
+};
+
+struct Plan {
+    Context* ctx = nullptr;
+    Scene* scene = nullptr;
+    vlb_bake_params params{};
+    std::vector<vlb_light> lights;
+    int K = 9;
+    int precision = VLB_PREC_FP32;
+    BakeArgs args{};
+    MergeArgs margs{};
+    int grid = 0;
+    DevBuf probes, dirs, ytab, partials, spill, out, out64, probe_perm, dir_perm, counters;
+    bool timing = false;
+    bool use_graph = false;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    bool ran = false;
+    vlb_bake_stats stats{};
+    ~Plan();
+};
+
+// ----------------------------------------------------------- launchers
+// bvh.hip
+void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
+                     const uint32_t* idx, uint32_t ntris, const float* normals,
+                     const float* albedo);
+// bake.hip
+int  bake_grid_size(Context& ctx, int precision);
+void launch_bake(const BakeArgs& a, int precision, bool record, bool count, int grid,
+                 hipStream_t stream);
+void launch_merge(const MergeArgs& m, hipStream_t stream);
+void launch_trace_closest(const SceneView& sc, const float* org, const float* dir, int n,
+                          float tmin, float tmax, int* prim, float* t, float* u, float* v,
+                          uint32_t* spill, uint32_t spill_stride, int grid, hipStream_t st);
+void launch_trace_occluded(const SceneView& sc, const float* org, const float* dir,
+                           const float* tmax, int n, float tmin, uint8_t* occ,
+                           uint32_t* spill, uint32_t spill_stride, int grid, hipStream_t st);
+// sh_image.hip
+void launch_skybox_sh(const float4* cubes, int N, int batch, int K, const float* wtab,
+                      float* partial, int blocks_per_cube, float* out, hipStream_t st,
+                      hipEvent_t e0, hipEvent_t e1, hipEvent_t e2);
+void launch_equirect_project(const uint8_t* img, int W, int H, int variant, double* out,
+                             float* partial, int nblocks, hipStream_t st);
+void launch_sh_reconstruct(const float* coeffs, int W, int H, uint8_t* out, hipStream_t st);
+
+// host_util.cpp
+void fibonacci_dirs(uint32_t R, float* xyz);
+void equirect_dirs(uint32_t W, uint32_t H, float* xyz, float* weight);
+void sh_eval_double(int K, bool exact, const double d[3], double* out);
+void geometric_normal(const float* v0, const float* v1, const float* v2, float g[3]);
+
+}  // namespace vlb
