@@ -1,0 +1,281 @@
+#!/usr/bin/env python3
+"""Benchmark of the probe-baking hot path on one MI355X.
+
+Default workload (the plain single-scene bake): procedural room 10x4x10 m
+(64 boxes + 8 icospheres, 11,018 triangles, seed 2), 16x8x16 = 2048 probes,
+4096 Fibonacci rays per probe, one directional light + procedural sky, one
+shadow ray per lit hit, L2 SH (9 coefficients x RGB), FP32 projection.
+
+A step is one full bake of that grid with every input resident in HBM: the
+fused trace/shade/project kernel plus the FP64 merge kernel (vlb_plan_execute).
+`value` is probes/s over the K timed steps (barrier + device sync on both
+sides).  The roofline figure is for the dominant kernel (k_bake), timed with
+hipEvents on the library's own stream inside the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--precision fp32|bf16|fp16]
+                    [--config c2|c1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "vulkan-light-bakery_amd"))
+
+METRIC = "probes/sec & Grays/s at ≤1e-3 rel SH err vs FP64; p50/p99 rebake latency"
+HBM_PEAK_GBS = 8000.0
+PREC = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16_fast": 3, "fp16_fast": 4}
+
+
+def rel_err(a, b):
+    import numpy as np
+
+    num = np.sqrt(((np.asarray(a, np.float64) - b) ** 2).sum(axis=1))
+    den = np.sqrt((np.asarray(b, np.float64) ** 2).sum(axis=1))
+    return num / np.maximum(den, 1e-30)
+
+
+def load_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def bench_c2(args):
+    import numpy as np
+
+    import vlb
+
+    scene = vlb.generate_scene(vlb.SCENE_ROOM, 2)
+    params = vlb.config_c2_params(scene, seed=2, precision=PREC[args.precision])
+    params.trace_variant = args.variant
+    P = params.probes.shape[0]
+    R = params.rays
+    ctx = vlb.Context(0)
+    sid = ctx.upload_scene_dict(scene)
+    info = ctx.scene_info(sid)
+    plan = ctx.plan(sid, params)
+    # one counting pass (outside the timed region): node visits, triangle
+    # tests and traced shadow rays of exactly this bake
+    plan.count()
+    cnt = plan.stats()
+    for _ in range(args.warmup):
+        plan.execute()
+    plan.sync()
+    plan.set_timing(True)
+    plan.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.execute()
+    plan.sync()
+    t1 = time.perf_counter()
+    st = plan.stats()
+    coeffs = plan.download()
+    elapsed = t1 - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    probes_per_s = P * args.steps / elapsed
+    rays_per_step = cnt["primary_rays"] + cnt["shadow_rays"]
+    grays = rays_per_step * args.steps / elapsed / 1e9
+
+    # algorithmic bytes of one k_bake launch: 64 B per inner node fetched +
+    # 48 B per triangle tested (+ the 48 B shading record per primary hit is
+    # not counted), both counted by the kernel's counting variant
+    alg_bytes = 64.0 * cnt["node_visits"] + 48.0 * cnt["tri_tests"]
+    trace_s = st["trace_ms"] / 1e3
+    achieved = alg_bytes / trace_s / 1e9
+    traffic = load_traffic("bake_c2_traffic.json")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None}
+
+    # error of this run against the FP64 oracle on a seeded probe subset
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+
+    rng = np.random.default_rng(0)
+    sub = np.sort(rng.choice(P, size=64, replace=False))
+    psub = vlb.config_c2_params(scene, seed=2)
+    psub.probes = params.probes[sub].copy()
+    oscene = orc.Scene.from_dict(scene)
+    ref = oscene.bake(psub, mode64=True, nthreads=min(16, os.cpu_count() or 1))["coeffs"]
+    err = rel_err(coeffs[sub], ref)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the FP32 C oracle on the same scene, probes and rays: whole c2
+        # bakes repeated until >= 10 s of wall time on the host's cores
+        nthreads = min(16, os.cpu_count() or 1)
+        pc = vlb.config_c2_params(scene, seed=2)
+        reps = 0
+        c0 = time.perf_counter()
+        while True:
+            oscene.bake(pc, mode64=False, nthreads=nthreads)
+            reps += 1
+            if time.perf_counter() - c0 >= 10.0:
+                break
+        c1 = time.perf_counter()
+        cpu = {"value": round(reps * P / (c1 - c0), 2), "unit": "probes/s", "cores": nthreads,
+               "kind": "port",
+               "sample": f"FP32 C oracle: {reps} full bakes of the same {P} probes x {R} rays "
+                         f"(same scene, BVH-independent results), {nthreads} threads, {c1 - c0:.1f} s"}
+
+    line = {
+        "metric": METRIC,
+        "value": round(probes_per_s, 2),
+        "unit": "probes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else args.precision,
+        "data": "synthetic (seeded procedural room, Fibonacci rays)",
+        "config": {"workload": "c2 plain single-scene bake", "scene": "room seed 2",
+                   "triangles": info["num_tris"], "bvh_depth": info["depth"], "probes": P,
+                   "rays_per_probe": R, "sh_order": 2, "lights": "1 directional + sky",
+                   "precision": args.precision},
+        "grays_per_s": round(grays, 3),
+        "probe_rays_per_s": round(P * R * args.steps / elapsed, 1),
+        "shadow_rays_per_step": cnt["shadow_rays"],
+        "kernel_ms": {"trace": round(st["trace_ms"], 4), "project": round(st["project_ms"], 4),
+                      "merge": round(st["merge_ms"], 4), "timed_runs": st["timed_runs"]},
+        "sh_rel_err_vs_fp64": {"max": float(err.max()), "p99": float(np.percentile(err, 99)),
+                               "probes": len(sub)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    plan.destroy()
+    ctx.release_scene(sid)
+    ctx.close()
+    return line
+
+
+def bench_c1(args):
+    """Skybox SH: 1024 procedural 6x128x128 RGBA32F cubemaps -> L2 per step."""
+    import numpy as np
+
+    import vlb
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+
+    N, B = 128, 1024
+    rng = np.random.default_rng(1)
+    base = np.stack([orc.procedural_cubemap(N, vlb.sun_direction(rng)) for _ in range(8)])
+    ctx = vlb.Context(0)
+    nbytes = B * 6 * N * N * 16
+    d_in = ctx.alloc(nbytes)
+    d_out = ctx.alloc(B * 27 * 4)
+    for i in range(B // 8):
+        ctx.upload(d_in + i * base.nbytes, base)
+    for _ in range(args.warmup):
+        ctx.skybox_sh_device(d_in, N, B, 2, d_out)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.skybox_sh_device(d_in, N, B, 2, d_out)
+    ctx.sync()
+    t1 = time.perf_counter()
+    # kernel duration: events of the last step, then a few event-timed steps
+    proj = [ctx.skybox_last_ms()[0]]
+    for _ in range(4):
+        ctx.skybox_sh_device(d_in, N, B, 2, d_out)
+        proj.append(ctx.skybox_last_ms()[0])
+    out = np.zeros((B, 9, 3), np.float32)
+    ctx.download(d_out, out)
+    ref = orc.skybox_sh(base[0], 9)
+    err = rel_err(out[0], ref)
+    kms = float(np.mean(proj))
+    achieved = (nbytes + B * 27 * 4) / (kms / 1e3) / 1e9
+    line = {"metric": "skyboxes/s (6x128x128 RGBA32F -> L2)", "value": round(B * args.steps / (t1 - t0), 1),
+            "unit": "skyboxes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(1e3 * (t1 - t0) / args.steps, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "c1 skybox SH", "batch": B, "face": N},
+            "sh_rel_err_vs_fp64": float(err.max()),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+            "cpu_baseline": None}
+    ctx.close()
+    return line
+
+
+def sweep_c2(args):
+    """Trace-kernel variants on the c2 bake: one JSON line per variant."""
+    import vlb
+
+    scene = vlb.generate_scene(vlb.SCENE_ROOM, 2)
+    ctx = vlb.Context(0)
+    sid = ctx.upload_scene_dict(scene)
+    rows = []
+    for v in range(0, 10):
+        params = vlb.config_c2_params(scene, seed=2)
+        params.trace_variant = v
+        plan = ctx.plan(sid, params)
+        for _ in range(3):
+            plan.execute()
+        plan.set_timing(True)
+        for _ in range(args.steps):
+            plan.execute()
+        plan.sync()
+        st = plan.stats()
+        rows.append({"variant": v, "map": (v - 1) % 3 if v else 0, "occ": [0, 6, 8][(v - 1) // 3] if v else 0,
+                     "trace_ms": round(st["trace_ms"], 4), "project_ms": round(st["project_ms"], 4),
+                     "total_ms": round(st["total_ms"], 4), "grid": st["grid"]})
+        print(json.dumps(rows[-1]), flush=True)
+        plan.destroy()
+    ctx.close()
+    best = min(rows, key=lambda r: r["total_ms"])
+    return {"sweep": "c2 trace variants", "best": best}
+
+
+def bench_copy(args):
+    """Streaming copy of 1 GiB (k_copy): attainable HBM bandwidth, and the
+    reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
+    import vlb
+
+    ctx = vlb.Context(0)
+    nbytes = 1 << 30
+    src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    for _ in range(args.warmup):
+        ctx.device_copy(dst, src, nbytes)
+    ms = [ctx.device_copy(dst, src, nbytes) for _ in range(args.steps)]
+    best = min(ms)
+    mean = sum(ms) / len(ms)
+    line = {"metric": "streaming copy bandwidth (read+write)", "value": round(2 * nbytes / (mean / 1e3) / 1e9, 1),
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(mean, 4), "best_ms": round(best, 4), "bytes_copied": nbytes,
+            "higher_is_better": True, "dtype": "u8", "config": {"workload": "k_copy 1 GiB"}}
+    ctx.close()
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c2", choices=["c2", "c1", "copy", "sweep"])
+    ap.add_argument("--precision", default="fp32", choices=sorted(PREC))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variant", type=int, default=0, help="trace kernel variant (0 = default)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the probe baker has no multi-GPU path in this "
+              "build (probes shard trivially; see the design notes).  Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    line = {"c2": bench_c2, "c1": bench_c1, "copy": bench_copy, "sweep": sweep_c2}[args.config](args)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -55,7 +55,19 @@ enum {
 };
 
 /* ----------------------------------------------------------------- options */
-enum { VLB_PREC_FP32 = 0, VLB_PREC_BF16 = 1, VLB_PREC_FP16 = 2 };
+/* Arithmetic of the SH projection (hit search and shading are always FP32).
+ *  FP32      : v_mfma_f32_16x16x4_f32, FP32 operands.
+ *  BF16/FP16 : v_mfma_f32_16x16x32_{bf16,f16}, operands split into a high
+ *              and a low 16-bit part, three products (hi*hi + hi*lo + lo*hi)
+ *              accumulated in FP32: ~16-bit mantissa operands.
+ *  *_FAST    : one 16-bit MFMA product per chunk (8 / 11-bit mantissas);
+ *              systematic error of a few 1e-3 on flat-lit scenes, kept for
+ *              the precision study.
+ * FP16 forms scale radiance by 1/16 (and clamp to the FP16 range) before
+ * conversion; the FP64 merge undoes the scale.
+ * All forms accumulate chunks with FP32 Kahan and merge slices in FP64. */
+enum { VLB_PREC_FP32 = 0, VLB_PREC_BF16 = 1, VLB_PREC_FP16 = 2,
+       VLB_PREC_BF16_FAST = 3, VLB_PREC_FP16_FAST = 4 };
 
 /* Shading of a primary hit.
  *  LAMBERT   : albedo/pi * sum_lights E * max(N.L,0) * visibility (linear HDR),
@@ -151,8 +163,10 @@ typedef struct vlb_bake_params {
     vlb_sky          sky;
     vlb_ref_shading  ref;
     /* tuning (0 = automatic) */
-    uint32_t         rays_per_slice;
-    uint32_t         reserved[3];
+    uint32_t         rays_per_slice;  /* projection slice length (rays)     */
+    uint32_t         trace_variant;   /* 1 + map + 3*occ: lane->ray map 0..2 */
+                                      /* x waves/SIMD bound {none, 6, 8}     */
+    uint32_t         reserved[2];
 } vlb_bake_params;
 
 typedef struct vlb_scene_info {
@@ -171,13 +185,16 @@ typedef struct vlb_bake_stats {
     uint64_t shadow_rays;      /* shadow rays actually traced             */
     uint64_t node_visits;      /* inner nodes fetched (stats pass only)   */
     uint64_t tri_tests;        /* triangles tested (stats pass only)      */
-    double   trace_ms;         /* trace+shade+project kernel, last run    */
-    double   merge_ms;         /* FP64 merge kernel, last run             */
-    double   total_ms;         /* whole execute on the device, last run   */
+    double   trace_ms;         /* trace + shade kernel  \                  */
+    double   project_ms;       /* MFMA projection kernel > mean over the  */
+    double   merge_ms;         /* FP64 merge kernel     /  timed executes */
+    double   total_ms;         /* whole execute on the device             */
     uint32_t tiles;
     uint32_t grid;
     uint32_t rays_per_slice;
     uint32_t slices;
+    uint32_t timed_runs;       /* executes averaged (since set_timing(1)) */
+    uint32_t reserved;
 } vlb_bake_stats;
 
 /* ------------------------------------------------------------ context API */
@@ -237,7 +254,9 @@ vlb_status  vlb_plan_execute(vlb_plan* plan);
 vlb_status  vlb_plan_sync(vlb_plan* plan);
 vlb_status  vlb_plan_download(vlb_plan* plan, float* out_coeffs, size_t out_len);
 vlb_status  vlb_plan_download_f64(vlb_plan* plan, double* out_coeffs, size_t out_len);
-/* Enable per-kernel hipEvent timing in execute (adds event records). */
+/* Enable per-kernel hipEvent timing in execute: every execute records event
+ * pairs on the context stream around the bake and merge kernels; stats()
+ * reports the means over the executes since the last set_timing(plan, 1). */
 vlb_status  vlb_plan_set_timing(vlb_plan* plan, int32_t on);
 /* Capture execute() as a hipGraph and replay it from then on. */
 vlb_status  vlb_plan_use_graph(vlb_plan* plan, int32_t on);
@@ -289,6 +308,12 @@ vlb_status  vlb_device_free(vlb_ctx* ctx, void* dptr);
 vlb_status  vlb_upload(vlb_ctx* ctx, void* dptr, const void* src, size_t bytes);
 vlb_status  vlb_download(vlb_ctx* ctx, void* dst, const void* dptr, size_t bytes);
 vlb_status  vlb_ctx_sync(vlb_ctx* ctx);
+/* Streaming copy kernel (16-byte loads/stores) of `bytes` (multiple of 16)
+ * between device buffers, on the context stream.  No counterpart in the
+ * reference: it calibrates HBM byte counters and measures attainable
+ * bandwidth.  *ms (may be NULL) gets the event-timed duration. */
+vlb_status  vlb_device_copy(vlb_ctx* ctx, void* dst, const void* src, size_t bytes,
+                            double* ms);
 
 /* ------------------------------------------------------------ host helpers */
 /* LightBaker::probePositionsFromBoudingBox (src/baker/light_baker.cpp:80-101):

@@ -116,15 +116,19 @@ def test_bake_radiance_and_hits_match_oracle(ctx, room):
     np.testing.assert_allclose(rad, ref["radiance"], rtol=2e-6, atol=1e-7)
 
 
-@pytest.mark.parametrize("prec", [vlb.PREC_BF16, vlb.PREC_FP16])
-def test_bake_low_precision_within_1e3(ctx, room, prec):
+@pytest.mark.parametrize("prec,bound", [(vlb.PREC_BF16, 1e-4), (vlb.PREC_FP16, 1e-4),
+                                        (vlb.PREC_BF16_FAST, 2e-2), (vlb.PREC_FP16_FAST, 3e-3)])
+def test_bake_16bit_projection_error(ctx, room, prec, bound):
+    # split (hi/lo) 16-bit forms must meet the FP32 bar within 1e-4; the
+    # single-product forms carry the 16-bit rounding of Y00 and of flat-lit
+    # radiance runs (bounded, not averaged away)
     s, sid, o = room
     p = _bake_params(_probes(48), rays=2048, precision=prec)
     g = ctx.bake(sid, p)
     p64 = _bake_params(_probes(48), rays=2048)
     ref = o.bake(p64, mode64=True)["coeffs"]
     err = rel_err(g, ref)
-    assert err.max() <= 1e-3, f"max rel err {err.max():.3e}"
+    assert err.max() <= bound, f"max rel err {err.max():.3e}"
 
 
 def test_bake_l3_exact_and_literal(ctx, room):

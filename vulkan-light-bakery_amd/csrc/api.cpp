@@ -46,6 +46,34 @@ Plan::~Plan()
         if (e) (void)hipEventDestroy(e);
 }
 
+// Reads the event triples of the executes recorded since the last drain.
+static void drain_timing(Plan& pl)
+{
+    for (uint32_t i = 0; i < pl.timed_pending; ++i) {
+        hipEvent_t* e = &pl.ev[Plan::kEv * (size_t)i];
+        float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
+        VLB_HIP(hipEventElapsedTime(&d, e[0], e[3]));
+        if (!pl.use_graph) {
+            VLB_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+            VLB_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+            VLB_HIP(hipEventElapsedTime(&c, e[2], e[3]));
+        }
+        pl.sum_trace += a;
+        pl.sum_project += b;
+        pl.sum_merge += c;
+        pl.sum_total += d;
+    }
+    pl.timed_done += pl.timed_pending;
+    pl.timed_pending = 0;
+    if (pl.timed_done) {
+        pl.stats.trace_ms = pl.sum_trace / pl.timed_done;
+        pl.stats.project_ms = pl.sum_project / pl.timed_done;
+        pl.stats.merge_ms = pl.sum_merge / pl.timed_done;
+        pl.stats.total_ms = pl.sum_total / pl.timed_done;
+    }
+    pl.stats.timed_runs = pl.timed_done;
+}
+
 }  // namespace vlb
 
 namespace {
@@ -135,6 +163,19 @@ uint16_t to_fp16(float f)
     memcpy(&u, &h, 2);
     return u;
 }
+double from_bf16(uint16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+double from_fp16(uint16_t u)
+{
+    _Float16 h;
+    memcpy(&h, &u, 2);
+    return (double)(float)h;
+}
 
 void validate(const vlb_bake_params& p)
 {
@@ -143,7 +184,7 @@ void validate(const vlb_bake_params& p)
     VLB_REQUIRE(p.sh_order == 2 || p.sh_order == 3, VLB_ERR_INVALID, "sh_order must be 2 or 3");
     VLB_REQUIRE(p.sh_constants == VLB_SH_LITERAL || p.sh_constants == VLB_SH_EXACT, VLB_ERR_INVALID,
                 "bad sh_constants");
-    VLB_REQUIRE(p.precision >= VLB_PREC_FP32 && p.precision <= VLB_PREC_FP16, VLB_ERR_INVALID,
+    VLB_REQUIRE(p.precision >= VLB_PREC_FP32 && p.precision <= VLB_PREC_FP16_FAST, VLB_ERR_INVALID,
                 "bad precision");
     VLB_REQUIRE(p.shade_mode == VLB_SHADE_LAMBERT || p.shade_mode == VLB_SHADE_REFERENCE,
                 VLB_ERR_INVALID, "bad shade_mode");
@@ -199,14 +240,19 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     const std::vector<uint32_t> dorder = morton_order(dxyz.data(), R);
 
     // ---- tiling
-    pl.grid = bake_grid_size(ctx, p.precision);
+    // tuning: trace_variant = 1 + map + 3 * occ_index (0 = default)
+    const uint32_t tv = p.trace_variant;
+    const int occ_tab[3] = {0, 6, 8};
+    pl.occ = tv ? occ_tab[((tv - 1) / 3) % 3] : 0;
+    pl.grid = trace_grid_size(ctx, pl.occ);
+    pl.pgrid = ctx.num_cus * 8;
     const int npg = (int)((P + 15) / 16);
     const int Ppad = npg * 16;
     uint32_t RS;
     if (p.rays_per_slice) {
         RS = (p.rays_per_slice + 31) & ~31u;
     } else {
-        uint64_t nsl = ((uint64_t)4 * pl.grid + npg - 1) / npg;
+        uint64_t nsl = ((uint64_t)2 * pl.pgrid + npg - 1) / npg;
         const uint64_t max_sl = (R + 31) / 32;
         nsl = std::max<uint64_t>(1, std::min(nsl, max_sl));
         RS = (uint32_t)(((R + nsl - 1) / nsl + 31) & ~31ull);
@@ -214,14 +260,19 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     const int nslices = (int)((R + RS - 1) / RS);
     const int Rpad = nslices * (int)RS;
     VLB_REQUIRE((uint64_t)npg * nslices < (1ull << 31), VLB_ERR_INVALID, "too many tiles");
+    VLB_REQUIRE((uint64_t)npg * (Rpad / 32) * 512 < (1ull << 40), VLB_ERR_INVALID, "bake too large");
 
     // ---- direction + SH tables in sorted order
     std::vector<float4> dirs4(Rpad, make_float4(0.f, 0.f, 0.f, 0.f));
     std::vector<int> dperm(Rpad, 0);
+    // 16-bit forms: [hi table][lo table] (lo only for the split forms)
+    const bool is16 = p.precision != VLB_PREC_FP32;
+    const bool split = p.precision == VLB_PREC_BF16 || p.precision == VLB_PREC_FP16;
+    const bool bf = p.precision == VLB_PREC_BF16 || p.precision == VLB_PREC_BF16_FAST;
     std::vector<float> yt32;
     std::vector<uint16_t> yt16;
-    if (p.precision == VLB_PREC_FP32) yt32.assign((size_t)Rpad * 16, 0.0f);
-    else yt16.assign((size_t)Rpad * 16, 0);
+    if (!is16) yt32.assign((size_t)Rpad * 16, 0.0f);
+    else yt16.assign((size_t)Rpad * 16 * (split ? 2 : 1), 0);
     const bool exact = p.sh_constants == VLB_SH_EXACT;
     for (int r = 0; r < Rpad; ++r) {
         if (r >= (int)R) continue;
@@ -235,10 +286,17 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
         const double wr = (double)dw[c] / wref;
         for (int k = 0; k < K; ++k) {
             const size_t ti = ((size_t)(r >> 3) * 16 + k) * 8 + (r & 7);
-            const float v = (float)(Y[k] * wr);
-            if (p.precision == VLB_PREC_FP32) yt32[ti] = v;
-            else if (p.precision == VLB_PREC_BF16) yt16[ti] = to_bf16(v);
-            else yt16[ti] = to_fp16(v);
+            const double v = Y[k] * wr;
+            if (!is16) {
+                yt32[ti] = (float)v;
+            } else {
+                const uint16_t hi = bf ? to_bf16((float)v) : to_fp16((float)v);
+                yt16[ti] = hi;
+                if (split) {
+                    const double rest = v - (bf ? from_bf16(hi) : from_fp16(hi));
+                    yt16[(size_t)Rpad * 16 + ti] = bf ? to_bf16((float)rest) : to_fp16((float)rest);
+                }
+            }
         }
     }
     // ---- probes in Morton order, padded to groups of 16
@@ -257,8 +315,9 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     pl.probe_perm = upload_vec(ctx, pperm);
     pl.dirs = upload_vec(ctx, dirs4);
     pl.dir_perm = upload_vec(ctx, dperm);
-    if (p.precision == VLB_PREC_FP32) pl.ytab = upload_vec(ctx, yt32);
+    if (!is16) pl.ytab = upload_vec(ctx, yt32);
     else pl.ytab = upload_vec(ctx, yt16);
+    pl.rad = dev_alloc(ctx, (size_t)npg * (Rpad / 32) * 1536 * sizeof(float));
     pl.partials = dev_alloc(ctx, (size_t)nslices * Ppad * 48 * sizeof(float2));
     const uint32_t spill_stride = (uint32_t)pl.grid * 256u;
     const uint32_t spill_entries = sc.depth > (uint32_t)kLdsDepth ? sc.depth - kLdsDepth + 1 : 0;
@@ -289,6 +348,10 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     a.nslices = nslices;
     a.npg = npg;
     a.ntiles = npg * nslices;
+    a.nchunk = Rpad / 32;
+    a.ntrace = npg * a.nchunk;
+    a.rad = pl.rad.as<float>();
+    a.map = tv ? (int)((tv - 1) % 3) : 0;
     a.tmin = p.t_min;
     a.tmax = p.t_max;
     a.partials = pl.partials.as<float2>();
@@ -305,7 +368,9 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     m.Ppad = Ppad;
     m.K = K;
     m.probe_perm = pl.probe_perm.as<int>();
-    m.scale = wref;
+    // FP16 forms carry radiance / 16 (range headroom); undo it here.
+    const bool fp16 = p.precision == VLB_PREC_FP16 || p.precision == VLB_PREC_FP16_FAST;
+    m.scale = fp16 ? wref * 16.0 : wref;
     m.out = pl.out.as<float>();
     m.out64 = pl.out64.as<double>();
 
@@ -314,18 +379,22 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     pl.stats.grid = (uint32_t)pl.grid;
     pl.stats.rays_per_slice = RS;
     pl.stats.slices = (uint32_t)nslices;
-    for (auto& e : pl.ev) VLB_HIP(hipEventCreate(&e));
     VLB_HIP(hipStreamSynchronize(ctx.stream));
 }
+
+void plan_sync(Plan& pl);
 
 void plan_launch(Plan& pl)
 {
     hipStream_t st = pl.ctx->stream;
+    if (pl.timing && pl.timed_pending == (uint32_t)Plan::kTimingRing) plan_sync(pl);
+    hipEvent_t* ev = pl.timing ? &pl.ev[Plan::kEv * (size_t)pl.timed_pending] : nullptr;
     if (pl.use_graph) {
         if (!pl.graph_exec) {
             VLB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                launch_bake(pl.args, pl.precision, false, false, pl.grid, st);
+                launch_trace(pl.args, false, false, pl.occ, pl.grid, st);
+                launch_project(pl.args, pl.precision, pl.pgrid, st);
                 launch_merge(pl.margs, st);
             } catch (...) {
                 hipGraph_t g = nullptr;
@@ -336,33 +405,26 @@ void plan_launch(Plan& pl)
             VLB_HIP(hipStreamEndCapture(st, &pl.graph));
             VLB_HIP(hipGraphInstantiate(&pl.graph_exec, pl.graph, nullptr, nullptr, 0));
         }
-        if (pl.timing) VLB_HIP(hipEventRecord(pl.ev[0], st));
+        if (ev) VLB_HIP(hipEventRecord(ev[0], st));
         VLB_HIP(hipGraphLaunch(pl.graph_exec, st));
-        if (pl.timing) VLB_HIP(hipEventRecord(pl.ev[2], st));
+        if (ev) VLB_HIP(hipEventRecord(ev[3], st));
     } else {
-        if (pl.timing) VLB_HIP(hipEventRecord(pl.ev[0], st));
-        launch_bake(pl.args, pl.precision, false, false, pl.grid, st);
-        if (pl.timing) VLB_HIP(hipEventRecord(pl.ev[1], st));
+        if (ev) VLB_HIP(hipEventRecord(ev[0], st));
+        launch_trace(pl.args, false, false, pl.occ, pl.grid, st);
+        if (ev) VLB_HIP(hipEventRecord(ev[1], st));
+        launch_project(pl.args, pl.precision, pl.pgrid, st);
+        if (ev) VLB_HIP(hipEventRecord(ev[2], st));
         launch_merge(pl.margs, st);
-        if (pl.timing) VLB_HIP(hipEventRecord(pl.ev[2], st));
+        if (ev) VLB_HIP(hipEventRecord(ev[3], st));
     }
+    if (ev) ++pl.timed_pending;
     pl.ran = true;
 }
 
 void plan_sync(Plan& pl)
 {
     VLB_HIP(hipStreamSynchronize(pl.ctx->stream));
-    if (pl.timing && pl.ran) {
-        float ms = 0.f;
-        VLB_HIP(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[2]));
-        pl.stats.total_ms = ms;
-        if (!pl.use_graph) {
-            VLB_HIP(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[1]));
-            pl.stats.trace_ms = ms;
-            VLB_HIP(hipEventElapsedTime(&ms, pl.ev[1], pl.ev[2]));
-            pl.stats.merge_ms = ms;
-        }
-    }
+    if (pl.timed_pending) drain_timing(pl);
 }
 
 size_t scene_spill_bytes(const Scene& s, uint32_t stride)
@@ -664,8 +726,22 @@ vlb_status vlb_plan_download_f64(vlb_plan* h, double* out, size_t out_len)
 vlb_status vlb_plan_set_timing(vlb_plan* h, int32_t on)
 {
     if (!h) return VLB_ERR_INVALID;
-    reinterpret_cast<Plan*>(h)->timing = on != 0;
-    return VLB_OK;
+    Plan* pl = reinterpret_cast<Plan*>(h);
+    std::lock_guard<std::recursive_mutex> lk(pl->ctx->mtx);
+    return guard(pl->ctx, [&] {
+        plan_sync(*pl);
+        if (on && pl->ev.empty()) {
+            pl->ev.assign(Plan::kEv * (size_t)Plan::kTimingRing, nullptr);
+            for (auto& e : pl->ev) VLB_HIP(hipEventCreate(&e));
+        }
+        pl->timing = on != 0;
+        if (on) {
+            pl->timed_done = 0;
+            pl->sum_trace = pl->sum_project = pl->sum_merge = pl->sum_total = 0.0;
+            pl->stats.trace_ms = pl->stats.project_ms = pl->stats.merge_ms = pl->stats.total_ms = 0.0;
+            pl->stats.timed_runs = 0;
+        }
+    });
 }
 
 vlb_status vlb_plan_use_graph(vlb_plan* h, int32_t on)
@@ -683,7 +759,7 @@ vlb_status vlb_plan_count(vlb_plan* h)
     return guard(pl->ctx, [&] {
         hipStream_t st = pl->ctx->stream;
         VLB_HIP(hipMemsetAsync(pl->counters.p, 0, 4 * sizeof(unsigned long long), st));
-        launch_bake(pl->args, VLB_PREC_FP32, false, true, pl->grid, st);
+        launch_trace(pl->args, false, true, 0, pl->grid, st);
         unsigned long long cnt[4] = {0, 0, 0, 0};
         VLB_HIP(hipMemcpyAsync(cnt, pl->counters.p, sizeof cnt, hipMemcpyDeviceToHost, st));
         VLB_HIP(hipStreamSynchronize(st));
@@ -752,7 +828,7 @@ vlb_status vlb_bake_radiance(vlb_ctx* h, uint32_t scene, const vlb_bake_params* 
             d_hit = dev_alloc(*c, nr * sizeof(int));
             pl.args.hit_out = d_hit.as<int>();
         }
-        launch_bake(pl.args, VLB_PREC_FP32, true, false, pl.grid, c->stream);
+        launch_trace(pl.args, true, false, 0, pl.grid, c->stream);
         if (radiance)
             VLB_HIP(hipMemcpyAsync(radiance, d_rad.p, nr * 3 * sizeof(float), hipMemcpyDeviceToHost,
                                    c->stream));
@@ -942,6 +1018,26 @@ vlb_status vlb_download(vlb_ctx* h, void* dst, const void* dptr, size_t bytes)
     return guard(c, [&] {
         VLB_HIP(hipMemcpyAsync(dst, dptr, bytes, hipMemcpyDeviceToHost, c->stream));
         VLB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+vlb_status vlb_device_copy(vlb_ctx* h, void* dst, const void* src, size_t bytes, double* ms)
+{
+    if (!h || !dst || !src) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_REQUIRE(bytes % 16 == 0, VLB_ERR_INVALID, "copy size must be a multiple of 16");
+        VLB_HIP(hipSetDevice(c->device));
+        VLB_HIP(hipEventRecord(c->ev[0], c->stream));
+        launch_copy(dst, src, bytes, c->stream);
+        VLB_HIP(hipEventRecord(c->ev[1], c->stream));
+        if (ms) {
+            VLB_HIP(hipStreamSynchronize(c->stream));
+            float f = 0.f;
+            VLB_HIP(hipEventElapsedTime(&f, c->ev[0], c->ev[1]));
+            *ms = f;
+        }
     });
 }
 

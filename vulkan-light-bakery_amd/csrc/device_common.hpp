@@ -50,7 +50,7 @@ struct ShadeParams {
     vlb_ref_shading ref;
 };
 
-constexpr int kLdsDepth = 32;  // stack entries kept in LDS per lane
+constexpr int kLdsDepth = 16;  // stack entries kept in LDS per lane
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ float comp(const float3& v, int k)

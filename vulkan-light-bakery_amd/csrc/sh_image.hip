@@ -247,7 +247,24 @@ __global__ __launch_bounds__(256) void k_sh_recon(const float* __restrict__ coef
     reinterpret_cast<uchar4*>(out)[p] = o;
 }
 
+// Plain streaming copy, 16 B per lane per iteration (bandwidth reference).
+__global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst,
+                                              const float4* __restrict__ src, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
+{
+    const size_t n = bytes / 16;
+    if (n == 0) return;
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 256 * 64);
+    k_copy<<<grid, 256, 0, st>>>(static_cast<float4*>(dst), static_cast<const float4*>(src), n);
+    VLB_HIP(hipGetLastError());
+}
 
 void launch_skybox_sh(const float4* cubes, int N, int batch, int K, const float* wtab,
                       float* partial, int blocks_per_cube, float* out, hipStream_t st,

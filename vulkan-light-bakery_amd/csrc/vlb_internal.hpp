@@ -134,8 +134,12 @@ struct BakeArgs {
     const float4* probes;     // Ppad, w unused
     int P, Ppad;
     const float4* dirs;       // Rpad: xyz, w = 1 valid / 0 padding
-    const void* ytab;         // [Rpad/8][16][8] fp32 | bf16 | fp16
+    const void* ytab;         // [Rpad/8][16][8] fp32 | bf16 | fp16 (+ lo table)
     int R, Rpad, RS, nslices, npg, ntiles;
+    int nchunk;               // 32-ray chunks per probe group (Rpad / 32)
+    int ntrace;               // trace tiles = npg * nchunk
+    float* rad;               // [ntrace][3][64][8] radiance, MFMA A order
+    int map;                  // lane -> ray map of k_trace (0..2)
     float tmin, tmax;
     float2* partials;         // [nslices][Ppad][3][16] (sum, compensation)
     uint32_t* spill;
@@ -166,13 +170,21 @@ struct Plan {
     int precision = VLB_PREC_FP32;
     BakeArgs args{};
     MergeArgs margs{};
-    int grid = 0;
-    DevBuf probes, dirs, ytab, partials, spill, out, out64, probe_perm, dir_perm, counters;
+    int grid = 0;             // trace grid (persistent)
+    int occ = 0;              // k_trace waves-per-SIMD variant (0, 6, 8)
+    int pgrid = 0;            // projection grid
+    DevBuf probes, dirs, ytab, rad, partials, spill, out, out64, probe_perm, dir_perm, counters;
     bool timing = false;
     bool use_graph = false;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // timing: kEv events per execute, kTimingRing executes deep
+    static constexpr int kTimingRing = 512;
+    static constexpr int kEv = 4;
+    std::vector<hipEvent_t> ev;   // kEv * kTimingRing, created on set_timing
+    uint32_t timed_pending = 0;   // executes recorded but not yet read
+    uint32_t timed_done = 0;
+    double sum_trace = 0.0, sum_project = 0.0, sum_merge = 0.0, sum_total = 0.0;
     bool ran = false;
     vlb_bake_stats stats{};
     ~Plan();
@@ -184,9 +196,9 @@ void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
                      const uint32_t* idx, uint32_t ntris, const float* normals,
                      const float* albedo);
 // bake.hip
-int  bake_grid_size(Context& ctx, int precision);
-void launch_bake(const BakeArgs& a, int precision, bool record, bool count, int grid,
-                 hipStream_t stream);
+int  trace_grid_size(Context& ctx, int occ);
+void launch_trace(const BakeArgs& a, bool record, bool count, int occ, int grid, hipStream_t stream);
+void launch_project(const BakeArgs& a, int precision, int grid, hipStream_t stream);
 void launch_merge(const MergeArgs& m, hipStream_t stream);
 void launch_trace_closest(const SceneView& sc, const float* org, const float* dir, int n,
                           float tmin, float tmax, int* prim, float* t, float* u, float* v,
@@ -201,6 +213,7 @@ void launch_skybox_sh(const float4* cubes, int N, int batch, int K, const float*
 void launch_equirect_project(const uint8_t* img, int W, int H, int variant, double* out,
                              float* partial, int nblocks, hipStream_t st);
 void launch_sh_reconstruct(const float* coeffs, int W, int H, uint8_t* out, hipStream_t st);
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
 
 // host_util.cpp
 void fibonacci_dirs(uint32_t R, float* xyz);

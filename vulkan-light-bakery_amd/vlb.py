@@ -28,7 +28,7 @@ OK, ERR_INVALID, ERR_DEVICE, ERR_OOM, ERR_BUDGET, ERR_NOT_FOUND, ERR_BUSY, \
 STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "DEVICE", 3: "OOM", 4: "BUDGET", 5: "NOT_FOUND",
                 6: "BUSY", 7: "UNSUPPORTED", 8: "TIMEOUT", 9: "CANCELLED", 10: "INJECTED",
                 11: "NO_DEVICE"}
-PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2
+PREC_FP32, PREC_BF16, PREC_FP16, PREC_BF16_FAST, PREC_FP16_FAST = 0, 1, 2, 3, 4
 SHADE_LAMBERT, SHADE_REFERENCE = 0, 1
 DIRS_FIBONACCI, DIRS_EQUIRECT = 0, 1
 SH_LITERAL, SH_EXACT = 0, 1
@@ -65,7 +65,8 @@ class BakeParams(C.Structure):
                 ("shade_mode", C.c_int32), ("num_lights", C.c_uint32),
                 ("lights", C.POINTER(Light)), ("shadow_bias", C.c_float),
                 ("sky", Sky), ("ref", RefShading),
-                ("rays_per_slice", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("rays_per_slice", C.c_uint32), ("trace_variant", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class SceneInfo(C.Structure):
@@ -77,9 +78,10 @@ class SceneInfo(C.Structure):
 class BakeStats(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
-                ("trace_ms", C.c_double), ("merge_ms", C.c_double), ("total_ms", C.c_double),
+                ("trace_ms", C.c_double), ("project_ms", C.c_double), ("merge_ms", C.c_double),
+                ("total_ms", C.c_double),
                 ("tiles", C.c_uint32), ("grid", C.c_uint32), ("rays_per_slice", C.c_uint32),
-                ("slices", C.c_uint32)]
+                ("slices", C.c_uint32), ("timed_runs", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class HostScene(C.Structure):
@@ -130,6 +132,7 @@ def _load(path: str = LIB_PATH) -> C.CDLL:
         "vlb_upload": (I32, [VP, VP, VP, SZ]),
         "vlb_download": (I32, [VP, VP, VP, SZ]),
         "vlb_ctx_sync": (I32, [VP]),
+        "vlb_device_copy": (I32, [VP, VP, VP, SZ, P(C.c_double)]),
         "vlb_probe_grid": (I32, [F, F, P(U32), I32, F, F]),
         "vlb_directions": (I32, [I32, U32, U32, U32, F, F, SZ]),
         "vlb_base64_encode": (I32, [P(C.c_uint8), SZ, C.c_char_p, SZ, P(SZ)]),
@@ -263,6 +266,7 @@ class Params:
     ref_gloss: float = 16.0
     ref_flags: int = REF_SRGB | REF_UNORM8
     rays_per_slice: int = 0
+    trace_variant: int = 0
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -309,6 +313,7 @@ class Params:
         p.ref.glossiness = self.ref_gloss
         p.ref.flags = self.ref_flags
         p.rays_per_slice = self.rays_per_slice
+        p.trace_variant = self.trace_variant
         self._keep[:] = [pr, lights, p]
         return p
 
@@ -541,6 +546,12 @@ class Context:
 
     def sync(self):
         self._chk(lib.vlb_ctx_sync(self.h), "sync")
+
+    def device_copy(self, dst: int, src: int, nbytes: int) -> float:
+        ms = C.c_double(0)
+        self._chk(lib.vlb_device_copy(self.h, C.c_void_p(dst), C.c_void_p(src), nbytes,
+                                      C.byref(ms)), "device_copy")
+        return ms.value
 
     def __del__(self):
         try:
