@@ -61,8 +61,10 @@ def bench_c2(args):
     sid = ctx.upload_scene_dict(scene)
     info = ctx.scene_info(sid)
     plan = ctx.plan(sid, params)
-    # one counting pass (outside the timed region): node visits, triangle
+    # outside the timed region: the autotuner picks the trace-kernel variant
+    # on this workload, then one counting pass gives node visits, triangle
     # tests and traced shadow rays of exactly this bake
+    variant = plan.autotune(5) if args.variant == 0 else args.variant
     plan.count()
     cnt = plan.stats()
     for _ in range(args.warmup):
@@ -145,6 +147,10 @@ def bench_c2(args):
         "grays_per_s": round(grays, 3),
         "probe_rays_per_s": round(P * R * args.steps / elapsed, 1),
         "shadow_rays_per_step": cnt["shadow_rays"],
+        "per_ray": {"inner_nodes": round(cnt["node_visits"] / rays_per_step, 2),
+                    "triangles": round(cnt["tri_tests"] / rays_per_step, 2),
+                    "alg_bytes": round(alg_bytes / rays_per_step, 1)},
+        "trace_variant": variant,
         "kernel_ms": {"trace": round(st["trace_ms"], 4), "project": round(st["project_ms"], 4),
                       "merge": round(st["merge_ms"], 4), "timed_runs": st["timed_runs"]},
         "sh_rel_err_vs_fp64": {"max": float(err.max()), "p99": float(np.percentile(err, 99)),
@@ -216,7 +222,7 @@ def sweep_c2(args):
     ctx = vlb.Context(0)
     sid = ctx.upload_scene_dict(scene)
     rows = []
-    for v in range(0, 10):
+    for v in range(1, 25):
         params = vlb.config_c2_params(scene, seed=2)
         params.trace_variant = v
         plan = ctx.plan(sid, params)
@@ -227,7 +233,9 @@ def sweep_c2(args):
             plan.execute()
         plan.sync()
         st = plan.stats()
-        rows.append({"variant": v, "map": (v - 1) % 3 if v else 0, "occ": [0, 6, 8][(v - 1) // 3] if v else 0,
+        kind = "wave" if v >= 13 else "fused"
+        rows.append({"variant": v, "kind": kind, "map": ((v - 13) if v >= 13 else (v - 1)) % 4,
+                     "occ_or_refill": ([16, 32, 8][(v - 13) // 4] if v >= 13 else [0, 6, 8][(v - 1) // 4]),
                      "trace_ms": round(st["trace_ms"], 4), "project_ms": round(st["project_ms"], 4),
                      "total_ms": round(st["total_ms"], 4), "grid": st["grid"]})
         print(json.dumps(rows[-1]), flush=True)
@@ -235,6 +243,90 @@ def sweep_c2(args):
     ctx.close()
     best = min(rows, key=lambda r: r["total_ms"])
     return {"sweep": "c2 trace variants", "best": best}
+
+
+def bench_c3(args):
+    """Precision form: terrain + 50 icospheres (1.154 M tris, seed 3), 32x16x32
+    probes x 16384 Fibonacci rays, L3 in FP32 / BF16 / FP16 projection; per-band
+    error of every projection form against the FP64 oracle on a probe subset."""
+    import numpy as np
+
+    import vlb
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+
+    scene = vlb.generate_scene(vlb.SCENE_TERRAIN, 3)
+    ctx = vlb.Context(0)
+    sid = ctx.upload_scene_dict(scene)
+    info = ctx.scene_info(sid)
+    lo, hi = scene["bounds_min"], scene["bounds_max"]
+    probes, _ = vlb.probe_grid([4.0, lo[1] + 1.0, 4.0], [252.0, hi[1] + 20.0, 252.0], [32, 16, 32])
+    rng = np.random.default_rng(3)
+    sun = vlb.sun_direction(rng)
+
+    def params(prec, order, pr=probes):
+        return vlb.Params(probes=pr, rays=16384, sh_order=order, precision=prec,
+                          shade_mode=vlb.SHADE_LAMBERT, sun_dir=sun.tolist(),
+                          lights=[(vlb.LIGHT_DIRECTIONAL, (-sun).tolist(), (3.0, 3.0, 3.0))],
+                          shadow_bias=2e-3)
+
+    runs = {}
+    cnt = None
+    for name, prec in (("fp32", 0), ("bf16", 1), ("fp16", 2)):
+        plan = ctx.plan(sid, params(prec, 3))
+        variant = plan.autotune(1)
+        if cnt is None:
+            plan.count()
+            cnt = plan.stats()
+        plan.execute()
+        plan.sync()
+        plan.set_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.execute()
+        plan.sync()
+        t1 = time.perf_counter()
+        st = plan.stats()
+        runs[name] = {"probes_per_s": round(len(probes) * args.steps / (t1 - t0), 1),
+                      "ms_per_step": round(1e3 * (t1 - t0) / args.steps, 2),
+                      "trace_ms": round(st["trace_ms"], 3), "project_ms": round(st["project_ms"], 4),
+                      "merge_ms": round(st["merge_ms"], 4), "variant": variant}
+        plan.destroy()
+    # per-band error vs FP64 on a seeded probe subset (the oracle is the checker)
+    sub = np.sort(rng.choice(len(probes), size=24, replace=False))
+    osc = orc.Scene.from_dict(scene)
+    errs = {}
+    for order in (2, 3):
+        ref = osc.bake(params(0, order, probes[sub].copy()), mode64=True,
+                       nthreads=min(16, os.cpu_count() or 1))["coeffs"]
+        K = 9 if order == 2 else 16
+        for name, prec in (("fp32", 0), ("bf16", 1), ("fp16", 2), ("bf16_fast", 3), ("fp16_fast", 4)):
+            got = ctx.bake(sid, params(prec, order, probes[sub].copy()))
+            band = {}
+            for l in range(order + 1):
+                ks = slice(l * l, (l + 1) * (l + 1))
+                band[f"l{l}"] = float(rel_err(got[:, ks], ref[:, ks]).max())
+            band["all"] = float(rel_err(got, ref).max())
+            errs[f"L{order}_{name}"] = band
+    rays = cnt["primary_rays"] + cnt["shadow_rays"]
+    alg = 64.0 * cnt["node_visits"] + 48.0 * cnt["tri_tests"]
+    tr = runs["bf16"]["trace_ms"] / 1e3
+    line = {"metric": METRIC, "value": runs["bf16"]["probes_per_s"], "unit": "probes/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": 1, "ms_per_step": runs["bf16"]["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "data": "synthetic (seeded terrain + icospheres, Fibonacci rays)",
+            "config": {"workload": "c3 precision form", "triangles": info["num_tris"],
+                       "bvh_depth": info["depth"], "bvh_build_ms": round(info["build_ms"], 2),
+                       "probes": len(probes), "rays_per_probe": 16384, "sh_order": 3},
+            "runs": runs, "grays_per_s": round(rays / tr / 1e9, 3),
+            "per_ray": {"inner_nodes": round(cnt["node_visits"] / rays, 2),
+                        "triangles": round(cnt["tri_tests"] / rays, 2)},
+            "sh_rel_err_vs_fp64_by_band": errs, "error_probes": len(sub),
+            "roofline": {"bound": "hbm", "achieved": round(alg / tr / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(alg / tr / 1e9 / HBM_PEAK_GBS, 4), "traffic": None}}
+    ctx.close()
+    return line
 
 
 def bench_copy(args):
@@ -263,7 +355,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=["c2", "c1", "copy", "sweep"])
+    ap.add_argument("--config", default="c2", choices=["c2", "c1", "c3", "copy", "sweep"])
     ap.add_argument("--precision", default="fp32", choices=sorted(PREC))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="trace kernel variant (0 = default)")
@@ -273,7 +365,8 @@ def main():
               "build (probes shard trivially; see the design notes).  Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
-    line = {"c2": bench_c2, "c1": bench_c1, "copy": bench_copy, "sweep": sweep_c2}[args.config](args)
+    line = {"c2": bench_c2, "c1": bench_c1, "c3": bench_c3, "copy": bench_copy,
+            "sweep": sweep_c2}[args.config](args)
     print(json.dumps(line), flush=True)
 
 

@@ -164,8 +164,14 @@ typedef struct vlb_bake_params {
     vlb_ref_shading  ref;
     /* tuning (0 = automatic) */
     uint32_t         rays_per_slice;  /* projection slice length (rays)     */
-    uint32_t         trace_variant;   /* 1 + map + 3*occ: lane->ray map 0..2 */
-                                      /* x waves/SIMD bound {none, 6, 8}     */
+    uint32_t         trace_variant;   /* 1..12 one fused trace+shade kernel: */
+                                      /*   1 + map + 4*occ, lane->ray map    */
+                                      /*   0..3 x waves/SIMD {none, 6, 8};   */
+                                      /* 13..24 wavefront kernels (closest,  */
+                                      /*   shadow, shade): 13 + map +        */
+                                      /*   4*refill, refill {16, 32, 8}      */
+                                      /*   idle lanes; 0 = autotuned choice, */
+                                      /*   else 20                           */
     uint32_t         reserved[2];
 } vlb_bake_params;
 
@@ -263,6 +269,11 @@ vlb_status  vlb_plan_use_graph(vlb_plan* plan, int32_t on);
 /* Runs a counting variant of the trace kernel once (node visits, triangle
  * tests, shadow rays) and fills the counters of vlb_bake_stats. */
 vlb_status  vlb_plan_count(vlb_plan* plan);
+/* Autotuner: times the trace-kernel variants (lane->ray maps x occupancy
+ * bounds) on this plan's real workload, `iters` runs each, keeps the fastest
+ * for the plan and remembers it for later plans on the same scene and ray
+ * count.  *chosen (may be NULL) receives the trace_variant picked. */
+vlb_status  vlb_plan_autotune(vlb_plan* plan, uint32_t iters, uint32_t* chosen);
 vlb_status  vlb_plan_stats(vlb_plan* plan, vlb_bake_stats* out);
 vlb_status  vlb_plan_destroy(vlb_plan* plan);
 

@@ -239,3 +239,71 @@ def test_plan_graph_replays_match_eager(ctx, room):
         assert np.array_equal(got, eager)
         ctx.sync()
     pl.destroy()
+
+
+# ---- every trace pipeline computes the same bits: fused k_trace (1..12) and
+# the wavefront k_rays/k_shade pipeline (13..24)
+VARIANTS = [1, 3, 4, 8, 13, 15, 16, 20, 23]
+
+
+def test_trace_variants_are_bitwise_identical(ctx, room):
+    s, sid, o = room
+    p = _bake_params(_probes(40, seed=21), rays=700)
+    ref = None
+    for v in VARIANTS:
+        p.trace_variant = v
+        g = ctx.bake(sid, p)
+        if ref is None:
+            ref = g
+            assert rel_err(g, o.bake(p, mode64=True)["coeffs"]).max() <= 1e-5
+        assert np.array_equal(g.view(np.uint32), ref.view(np.uint32)), v
+
+
+@pytest.mark.parametrize("variant", [15, 16])
+def test_wavefront_radiance_and_hits(ctx, room, variant):
+    s, sid, o = room
+    p = _bake_params(_probes(20, seed=9), rays=256, trace_variant=variant)
+    rad, hits = ctx.bake_radiance(sid, p)
+    ref = o.bake(p, mode64=False, want_radiance=True, want_hits=True)
+    assert np.array_equal(hits, ref["hits"])
+    np.testing.assert_allclose(rad, ref["radiance"], rtol=2e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("P,R", [(1, 1), (17, 31), (33, 100)])
+def test_wavefront_ragged_and_reference(ctx, room, P, R):
+    s, sid, o = room
+    for mode in (vlb.SHADE_LAMBERT, vlb.SHADE_REFERENCE):
+        p = _bake_params(_probes(P, seed=P * R), rays=R, shade_mode=mode, ref_light=(5.0, 3.5, 5.0),
+                         trace_variant=16)
+        g = ctx.bake(sid, p)
+        assert rel_err(g, o.bake(p, mode64=True)["coeffs"]).max() <= 1e-5
+
+
+def test_wavefront_empty_scene_and_no_lights(ctx):
+    p = _bake_params(_probes(5), rays=300, trace_variant=15)
+    sid = ctx.upload_scene(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+    g = ctx.bake(sid, p)
+    ref = orc.Scene(np.zeros((0, 3)), np.zeros((0, 3), np.uint32)).bake(p)["coeffs"]
+    assert rel_err(g, ref).max() <= 1e-5
+    ctx.release_scene(sid)
+
+
+def test_wavefront_no_lights(ctx, room):
+    s, sid, o = room
+    p = _bake_params(_probes(8), rays=256, lights=[], trace_variant=16)
+    g = ctx.bake(sid, p)
+    assert rel_err(g, o.bake(p, mode64=True)["coeffs"]).max() <= 1e-5
+
+
+def test_autotune_picks_a_valid_variant_and_keeps_results(ctx, room):
+    s, sid, o = room
+    p = _bake_params(_probes(64, seed=3), rays=1024)
+    pl = ctx.plan(sid, p)
+    pl.execute()
+    before = pl.download()
+    v = pl.autotune(1)
+    assert 1 <= v <= 24
+    pl.execute()
+    after = pl.download()
+    assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
+    pl.destroy()

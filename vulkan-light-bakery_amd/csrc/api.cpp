@@ -78,6 +78,78 @@ static void drain_timing(Plan& pl)
 
 namespace {
 
+// trace_variant: 1..12 fused k_trace (1 + map + 4 * occupancy index),
+// 13..24 wavefront k_rays/k_shade (13 + map + 4 * refill index)
+constexpr uint32_t kDefaultVariant = 20;  // wavefront, lane map 3, refill at 32 idle lanes
+constexpr uint32_t kMaxVariant = 24;
+constexpr int kOccTab[3] = {0, 6, 8};
+constexpr int kRefillTab[3] = {16, 32, 8};
+
+struct Variant {
+    bool wave;
+    int map, occ, refill;
+};
+Variant decode_variant(uint32_t v)
+{
+    Variant d{};
+    if (v >= 13) {
+        d.wave = true;
+        d.map = (int)((v - 13) % 4);
+        d.refill = kRefillTab[((v - 13) / 4) % 3];
+    } else {
+        d.wave = false;
+        d.map = (int)((v - 1) % 4);
+        d.occ = kOccTab[((v - 1) / 4) % 3];
+        d.refill = 16;
+    }
+    return d;
+}
+
+// Allocates the wavefront buffers of a plan on first use.
+void ensure_wave_buffers(Context& ctx, Plan& pl)
+{
+    if (pl.hits.p) return;
+    const size_t nslots = (size_t)pl.args.ntrace * 512;
+    pl.hits = dev_alloc(ctx, nslots * sizeof(int4));
+    pl.vis = dev_alloc(ctx, std::max<size_t>(16, nslots * (size_t)pl.args.nvis));
+    pl.wctr = dev_alloc(ctx, 16 * sizeof(unsigned long long));
+    pl.args.hits = pl.hits.as<int4>();
+    pl.args.vis = pl.vis.as<uint8_t>();
+    pl.args.wctr = pl.wctr.as<unsigned long long>();
+}
+
+// Applies a trace_variant to a plan (launch shapes and lane map).
+void apply_variant(Context& ctx, Plan& pl, uint32_t v)
+{
+    const Variant d = decode_variant(v);
+    pl.variant = v;
+    pl.wave = d.wave;
+    pl.occ = d.occ;
+    pl.args.map = d.map;
+    pl.args.refill = d.refill;
+    if (d.wave) {
+        ensure_wave_buffers(ctx, pl);
+        pl.args.spill_stride = (uint32_t)pl.wgrid * 256u;
+    } else {
+        pl.grid = trace_grid_size(ctx, d.occ);
+        pl.args.spill_stride = (uint32_t)pl.grid * 256u;
+    }
+    pl.stats.grid = (uint32_t)(d.wave ? pl.wgrid : pl.grid);
+    if (pl.graph_exec) {  // launch shape changed: recapture on the next execute
+        (void)hipGraphExecDestroy(pl.graph_exec);
+        (void)hipGraphDestroy(pl.graph);
+        pl.graph_exec = nullptr;
+        pl.graph = nullptr;
+    }
+}
+
+// The trace + shade stage of the plan's variant.
+void launch_trace_stage(Plan& pl, bool record, bool count, hipStream_t st)
+{
+    if (pl.wave) launch_wave(pl.args, record, count, pl.wgrid, st);
+    else launch_trace(pl.args, record, count, record || count ? 0 : pl.occ, pl.grid, st);
+}
+
 template <class F>
 vlb_status guard(Context* ctx, F&& f)
 {
@@ -106,106 +178,6 @@ Scene& find_scene(Context& ctx, uint32_t id)
     return *it->second;
 }
 
-template <class T>
-DevBuf upload_vec(Context& ctx, const std::vector<T>& v)
-{
-    DevBuf b = dev_alloc(ctx, v.size() * sizeof(T));
-    VLB_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx.stream));
-    return b;
-}
-
-uint32_t expand10(uint32_t v)
-{
-    v &= 1023u;
-    v = (v * 0x00010001u) & 0xFF0000FFu;
-    v = (v * 0x00000101u) & 0x0F00F00Fu;
-    v = (v * 0x00000011u) & 0xC30C30C3u;
-    v = (v * 0x00000005u) & 0x49249249u;
-    return v;
-}
-
-// Morton order of 3D points (stable on equal codes).
-std::vector<uint32_t> morton_order(const float* xyz, size_t n)
-{
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (size_t i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = std::min(lo[a], xyz[3 * i + a]);
-            hi[a] = std::max(hi[a], xyz[3 * i + a]);
-        }
-    std::vector<uint32_t> code(n), order(n);
-    for (size_t i = 0; i < n; ++i) {
-        uint32_t q[3];
-        for (int a = 0; a < 3; ++a) {
-            float ext = hi[a] - lo[a];
-            float x = ext > 0.0f ? (xyz[3 * i + a] - lo[a]) / ext * 1023.0f : 0.0f;
-            q[a] = (uint32_t)std::min(1023.0f, std::max(0.0f, x));
-        }
-        code[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
-    }
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
-    return order;
-}
-
-uint16_t to_bf16(float f)
-{
-    uint32_t u;
-    memcpy(&u, &f, 4);
-    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)(u >> 16);  // inf / nan
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-uint16_t to_fp16(float f)
-{
-    _Float16 h = (_Float16)f;
-    uint16_t u;
-    memcpy(&u, &h, 2);
-    return u;
-}
-double from_bf16(uint16_t h)
-{
-    uint32_t u = (uint32_t)h << 16;
-    float f;
-    memcpy(&f, &u, 4);
-    return f;
-}
-double from_fp16(uint16_t u)
-{
-    _Float16 h;
-    memcpy(&h, &u, 2);
-    return (double)(float)h;
-}
-
-void validate(const vlb_bake_params& p)
-{
-    VLB_REQUIRE(p.num_probes >= 1 && p.probe_xyz, VLB_ERR_INVALID, "num_probes must be >= 1 with probe_xyz");
-    VLB_REQUIRE(p.num_probes <= (1u << 26), VLB_ERR_INVALID, "num_probes too large");
-    VLB_REQUIRE(p.sh_order == 2 || p.sh_order == 3, VLB_ERR_INVALID, "sh_order must be 2 or 3");
-    VLB_REQUIRE(p.sh_constants == VLB_SH_LITERAL || p.sh_constants == VLB_SH_EXACT, VLB_ERR_INVALID,
-                "bad sh_constants");
-    VLB_REQUIRE(p.precision >= VLB_PREC_FP32 && p.precision <= VLB_PREC_FP16_FAST, VLB_ERR_INVALID,
-                "bad precision");
-    VLB_REQUIRE(p.shade_mode == VLB_SHADE_LAMBERT || p.shade_mode == VLB_SHADE_REFERENCE,
-                VLB_ERR_INVALID, "bad shade_mode");
-    VLB_REQUIRE(p.num_lights <= VLB_MAX_LIGHTS, VLB_ERR_INVALID, "too many lights");
-    VLB_REQUIRE(p.num_lights == 0 || p.lights, VLB_ERR_INVALID, "lights pointer is NULL");
-    for (uint32_t i = 0; i < p.num_lights; ++i)
-        VLB_REQUIRE(p.lights[i].type == VLB_LIGHT_DIRECTIONAL || p.lights[i].type == VLB_LIGHT_POINT,
-                    VLB_ERR_INVALID, "bad light type");
-    if (p.dir_mode == VLB_DIRS_FIBONACCI) {
-        VLB_REQUIRE(p.rays_per_probe >= 1 && p.rays_per_probe <= (1u << 26), VLB_ERR_INVALID,
-                    "rays_per_probe out of range");
-    } else if (p.dir_mode == VLB_DIRS_EQUIRECT) {
-        VLB_REQUIRE(p.equirect_w >= 1 && p.equirect_h >= 1 &&
-                        (uint64_t)p.equirect_w * p.equirect_h <= (1u << 26),
-                    VLB_ERR_INVALID, "equirect size out of range");
-    } else {
-        throw Error{VLB_ERR_INVALID, "bad dir_mode"};
-    }
-    VLB_REQUIRE(p.t_min >= 0.0f && p.t_max > p.t_min, VLB_ERR_INVALID, "need 0 <= t_min < t_max");
-    VLB_REQUIRE((uint64_t)p.num_probes * 48ull * 8ull < (1ull << 40), VLB_ERR_INVALID, "bake too large");
-}
 
 void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
 {
@@ -221,107 +193,37 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     const int K = pl.K;
     const uint32_t P = p.num_probes;
 
-    // ---- direction set (host, deterministic; same formulas as the oracle)
-    uint32_t R;
-    std::vector<float> dxyz, dw;
-    if (p.dir_mode == VLB_DIRS_FIBONACCI) {
-        R = p.rays_per_probe;
-        dxyz.resize((size_t)R * 3);
-        dw.assign(R, (float)(4.0 * M_PI / R));
-        fibonacci_dirs(R, dxyz.data());
-    } else {
-        R = p.equirect_w * p.equirect_h;
-        dxyz.resize((size_t)R * 3);
-        dw.resize(R);
-        equirect_dirs(p.equirect_w, p.equirect_h, dxyz.data(), dw.data());
+    // ---- tuning: trace_variant 0 = the autotuner's earlier choice for this
+    // scene and ray count, else the default
+    const uint32_t R = rays_of(p);
+    uint32_t tv = p.trace_variant;
+    if (tv == 0) {
+        auto it = ctx.tuned.find({sc.id, R});
+        tv = it != ctx.tuned.end() ? it->second : kDefaultVariant;
     }
-    const double wref = 4.0 * M_PI / R;
-    // spatial order of directions: neighbouring lanes trace neighbouring rays
-    const std::vector<uint32_t> dorder = morton_order(dxyz.data(), R);
-
-    // ---- tiling
-    // tuning: trace_variant = 1 + map + 3 * occ_index (0 = default)
-    const uint32_t tv = p.trace_variant;
-    const int occ_tab[3] = {0, 6, 8};
-    pl.occ = tv ? occ_tab[((tv - 1) / 3) % 3] : 0;
-    pl.grid = trace_grid_size(ctx, pl.occ);
+    VLB_REQUIRE(tv >= 1 && tv <= kMaxVariant, VLB_ERR_INVALID, "trace_variant out of range");
+    pl.grid = trace_grid_size(ctx, 0);
+    pl.wgrid = wave_grid_size(ctx);
+    const int max_grid = std::max(std::max(pl.grid, pl.wgrid), trace_grid_size(ctx, 8));
     pl.pgrid = ctx.num_cus * 8;
     const int npg = (int)((P + 15) / 16);
-    const int Ppad = npg * 16;
-    uint32_t RS;
-    if (p.rays_per_slice) {
-        RS = (p.rays_per_slice + 31) & ~31u;
-    } else {
-        uint64_t nsl = ((uint64_t)2 * pl.pgrid + npg - 1) / npg;
-        const uint64_t max_sl = (R + 31) / 32;
-        nsl = std::max<uint64_t>(1, std::min(nsl, max_sl));
-        RS = (uint32_t)(((R + nsl - 1) / nsl + 31) & ~31ull);
-    }
-    const int nslices = (int)((R + RS - 1) / RS);
-    const int Rpad = nslices * (int)RS;
+
+    // ---- direction + basis tables (sorted directions), probes (Morton order)
+    build_ray_table(ctx, p, K, slice_rays(p, R, npg, pl.pgrid), pl.rt);
+    build_probe_set(ctx, p.probe_xyz, P, pl.ps);
+    const int Ppad = pl.ps.Ppad;
+    const int nslices = pl.rt.nslices;
+    const int Rpad = (int)pl.rt.Rpad;
+    const double wref = pl.rt.wref;
     VLB_REQUIRE((uint64_t)npg * nslices < (1ull << 31), VLB_ERR_INVALID, "too many tiles");
     VLB_REQUIRE((uint64_t)npg * (Rpad / 32) * 512 < (1ull << 40), VLB_ERR_INVALID, "bake too large");
 
-    // ---- direction + SH tables in sorted order
-    std::vector<float4> dirs4(Rpad, make_float4(0.f, 0.f, 0.f, 0.f));
-    std::vector<int> dperm(Rpad, 0);
-    // 16-bit forms: [hi table][lo table] (lo only for the split forms)
-    const bool is16 = p.precision != VLB_PREC_FP32;
-    const bool split = p.precision == VLB_PREC_BF16 || p.precision == VLB_PREC_FP16;
-    const bool bf = p.precision == VLB_PREC_BF16 || p.precision == VLB_PREC_BF16_FAST;
-    std::vector<float> yt32;
-    std::vector<uint16_t> yt16;
-    if (!is16) yt32.assign((size_t)Rpad * 16, 0.0f);
-    else yt16.assign((size_t)Rpad * 16 * (split ? 2 : 1), 0);
-    const bool exact = p.sh_constants == VLB_SH_EXACT;
-    for (int r = 0; r < Rpad; ++r) {
-        if (r >= (int)R) continue;
-        const uint32_t c = dorder[r];
-        const float* d = &dxyz[3 * (size_t)c];
-        dirs4[r] = make_float4(d[0], d[1], d[2], 1.0f);
-        dperm[r] = (int)c;
-        double sh[3] = {d[0], d[2], d[1]};  // SH frame: world (x, z, y)
-        double Y[16];
-        sh_eval_double(K, exact, sh, Y);
-        const double wr = (double)dw[c] / wref;
-        for (int k = 0; k < K; ++k) {
-            const size_t ti = ((size_t)(r >> 3) * 16 + k) * 8 + (r & 7);
-            const double v = Y[k] * wr;
-            if (!is16) {
-                yt32[ti] = (float)v;
-            } else {
-                const uint16_t hi = bf ? to_bf16((float)v) : to_fp16((float)v);
-                yt16[ti] = hi;
-                if (split) {
-                    const double rest = v - (bf ? from_bf16(hi) : from_fp16(hi));
-                    yt16[(size_t)Rpad * 16 + ti] = bf ? to_bf16((float)rest) : to_fp16((float)rest);
-                }
-            }
-        }
-    }
-    // ---- probes in Morton order, padded to groups of 16
-    const std::vector<uint32_t> porder = morton_order(p.probe_xyz, P);
-    std::vector<float4> probes4(Ppad, make_float4(0.f, 0.f, 0.f, 0.f));
-    std::vector<int> pperm(Ppad, -1);
-    for (uint32_t i = 0; i < P; ++i) {
-        const float* q = &p.probe_xyz[3 * (size_t)porder[i]];
-        probes4[i] = make_float4(q[0], q[1], q[2], 1.0f);
-        pperm[i] = (int)porder[i];
-    }
-    for (int i = (int)P; i < Ppad; ++i) probes4[i] = probes4[P - 1];
-
     // ---- device buffers
-    pl.probes = upload_vec(ctx, probes4);
-    pl.probe_perm = upload_vec(ctx, pperm);
-    pl.dirs = upload_vec(ctx, dirs4);
-    pl.dir_perm = upload_vec(ctx, dperm);
-    if (!is16) pl.ytab = upload_vec(ctx, yt32);
-    else pl.ytab = upload_vec(ctx, yt16);
     pl.rad = dev_alloc(ctx, (size_t)npg * (Rpad / 32) * 1536 * sizeof(float));
     pl.partials = dev_alloc(ctx, (size_t)nslices * Ppad * 48 * sizeof(float2));
+    // spill area sized for the largest candidate grid of the autotuner
     const uint32_t spill_stride = (uint32_t)pl.grid * 256u;
-    const uint32_t spill_entries = sc.depth > (uint32_t)kLdsDepth ? sc.depth - kLdsDepth + 1 : 0;
-    pl.spill = dev_alloc(ctx, std::max<size_t>(16, (size_t)spill_entries * spill_stride * 4));
+    pl.spill = dev_alloc(ctx, scene_spill_bytes(sc, (uint32_t)max_grid * 256u));
     pl.out = dev_alloc(ctx, (size_t)P * K * 3 * sizeof(float));
     pl.out64 = dev_alloc(ctx, (size_t)P * K * 3 * sizeof(double));
     pl.counters = dev_alloc(ctx, 4 * sizeof(unsigned long long));
@@ -330,35 +232,29 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     BakeArgs& a = pl.args;
     memset(&a, 0, sizeof a);
     a.sc = sc.view();
-    a.sp.mode = p.shade_mode;
-    a.sp.nlights = (int)p.num_lights;
-    a.sp.shadow_bias = p.shadow_bias;
-    a.sp.t_max = p.t_max;
-    for (uint32_t i = 0; i < p.num_lights; ++i) a.sp.lights[i] = p.lights[i];
-    a.sp.sky = p.sky;
-    a.sp.ref = p.ref;
-    a.probes = pl.probes.as<float4>();
+    a.sp = shade_params(p);
+    a.probes = pl.ps.probes.as<float4>();
     a.P = (int)P;
     a.Ppad = Ppad;
-    a.dirs = pl.dirs.as<float4>();
-    a.ytab = pl.ytab.p;
+    a.dirs = pl.rt.dirs.as<float4>();
+    a.ytab = pl.rt.ytab.p;
     a.R = (int)R;
     a.Rpad = Rpad;
-    a.RS = (int)RS;
+    a.RS = (int)pl.rt.RS;
     a.nslices = nslices;
     a.npg = npg;
     a.ntiles = npg * nslices;
     a.nchunk = Rpad / 32;
     a.ntrace = npg * a.nchunk;
     a.rad = pl.rad.as<float>();
-    a.map = tv ? (int)((tv - 1) % 3) : 0;
+    a.nvis = p.shade_mode == VLB_SHADE_REFERENCE ? 1 : (int)p.num_lights;
     a.tmin = p.t_min;
     a.tmax = p.t_max;
     a.partials = pl.partials.as<float2>();
     a.spill = pl.spill.as<uint32_t>();
     a.spill_stride = spill_stride;
-    a.probe_perm = pl.probe_perm.as<int>();
-    a.dir_perm = pl.dir_perm.as<int>();
+    a.probe_perm = pl.ps.probe_perm.as<int>();
+    a.dir_perm = pl.rt.dir_perm.as<int>();
     a.counters = pl.counters.as<unsigned long long>();
 
     MergeArgs& m = pl.margs;
@@ -367,7 +263,7 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     m.P = (int)P;
     m.Ppad = Ppad;
     m.K = K;
-    m.probe_perm = pl.probe_perm.as<int>();
+    m.probe_perm = pl.ps.probe_perm.as<int>();
     // FP16 forms carry radiance / 16 (range headroom); undo it here.
     const bool fp16 = p.precision == VLB_PREC_FP16 || p.precision == VLB_PREC_FP16_FAST;
     m.scale = fp16 ? wref * 16.0 : wref;
@@ -376,8 +272,8 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
 
     pl.stats.primary_rays = (uint64_t)P * R;
     pl.stats.tiles = (uint32_t)a.ntiles;
-    pl.stats.grid = (uint32_t)pl.grid;
-    pl.stats.rays_per_slice = RS;
+    apply_variant(ctx, pl, tv);
+    pl.stats.rays_per_slice = pl.rt.RS;
     pl.stats.slices = (uint32_t)nslices;
     VLB_HIP(hipStreamSynchronize(ctx.stream));
 }
@@ -393,7 +289,7 @@ void plan_launch(Plan& pl)
         if (!pl.graph_exec) {
             VLB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                launch_trace(pl.args, false, false, pl.occ, pl.grid, st);
+                launch_trace_stage(pl, false, false, st);
                 launch_project(pl.args, pl.precision, pl.pgrid, st);
                 launch_merge(pl.margs, st);
             } catch (...) {
@@ -410,7 +306,7 @@ void plan_launch(Plan& pl)
         if (ev) VLB_HIP(hipEventRecord(ev[3], st));
     } else {
         if (ev) VLB_HIP(hipEventRecord(ev[0], st));
-        launch_trace(pl.args, false, false, pl.occ, pl.grid, st);
+        launch_trace_stage(pl, false, false, st);
         if (ev) VLB_HIP(hipEventRecord(ev[1], st));
         launch_project(pl.args, pl.precision, pl.pgrid, st);
         if (ev) VLB_HIP(hipEventRecord(ev[2], st));
@@ -425,12 +321,6 @@ void plan_sync(Plan& pl)
 {
     VLB_HIP(hipStreamSynchronize(pl.ctx->stream));
     if (pl.timed_pending) drain_timing(pl);
-}
-
-size_t scene_spill_bytes(const Scene& s, uint32_t stride)
-{
-    const uint32_t e = s.depth > (uint32_t)kLdsDepth ? s.depth - kLdsDepth + 1 : 0;
-    return std::max<size_t>(16, (size_t)e * stride * 4);
 }
 
 std::vector<float> solid_angle_table(int N)
@@ -759,13 +649,52 @@ vlb_status vlb_plan_count(vlb_plan* h)
     return guard(pl->ctx, [&] {
         hipStream_t st = pl->ctx->stream;
         VLB_HIP(hipMemsetAsync(pl->counters.p, 0, 4 * sizeof(unsigned long long), st));
-        launch_trace(pl->args, false, true, 0, pl->grid, st);
+        launch_trace_stage(*pl, false, true, st);
         unsigned long long cnt[4] = {0, 0, 0, 0};
         VLB_HIP(hipMemcpyAsync(cnt, pl->counters.p, sizeof cnt, hipMemcpyDeviceToHost, st));
         VLB_HIP(hipStreamSynchronize(st));
         pl->stats.node_visits = cnt[0];
         pl->stats.tri_tests = cnt[1];
         pl->stats.shadow_rays = cnt[2];
+    });
+}
+
+vlb_status vlb_plan_autotune(vlb_plan* h, uint32_t iters, uint32_t* chosen)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Plan* pl = reinterpret_cast<Plan*>(h);
+    Context& c = *pl->ctx;
+    std::lock_guard<std::recursive_mutex> lk(c.mtx);
+    return guard(&c, [&] {
+        hipStream_t st = c.stream;
+        iters = std::max(1u, std::min(iters, 50u));
+        hipEvent_t e0, e1;
+        VLB_HIP(hipEventCreate(&e0));
+        VLB_HIP(hipEventCreate(&e1));
+        // fused: lane maps 0..3 unbounded + maps 2, 3 at 6 waves/SIMD;
+        // wavefront: maps 0..3 with refill 16, maps 2, 3 with refill 32 / 8
+        const uint32_t cand[] = {1, 2, 3, 4, 7, 8, 13, 14, 15, 16, 19, 20, 23, 24};
+        double best = 1e30;
+        uint32_t best_v = pl->variant;
+        for (uint32_t v : cand) {
+            apply_variant(c, *pl, v);
+            launch_trace_stage(*pl, false, false, st);  // warm-up
+            VLB_HIP(hipEventRecord(e0, st));
+            for (uint32_t i = 0; i < iters; ++i) launch_trace_stage(*pl, false, false, st);
+            VLB_HIP(hipEventRecord(e1, st));
+            VLB_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            VLB_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_v = v;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        apply_variant(c, *pl, best_v);
+        c.tuned[{pl->scene->id, (uint32_t)pl->args.R}] = best_v;
+        if (chosen) *chosen = best_v;
     });
 }
 
@@ -828,7 +757,7 @@ vlb_status vlb_bake_radiance(vlb_ctx* h, uint32_t scene, const vlb_bake_params* 
             d_hit = dev_alloc(*c, nr * sizeof(int));
             pl.args.hit_out = d_hit.as<int>();
         }
-        launch_trace(pl.args, true, false, 0, pl.grid, c->stream);
+        launch_trace_stage(pl, true, false, c->stream);
         if (radiance)
             VLB_HIP(hipMemcpyAsync(radiance, d_rad.p, nr * 3 * sizeof(float), hipMemcpyDeviceToHost,
                                    c->stream));

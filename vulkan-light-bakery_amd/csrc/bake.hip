@@ -23,113 +23,19 @@
 // k_trace walks its rays XCD by XCD: workgroup b runs on XCD b % 8
 // (round-robin dispatch), and XCD x owns the x-th contiguous eighth of the
 // rays, so neighbouring probes and directions share that XCD's L2.
-#include "vlb_internal.hpp"
+#include "project_common.hpp"
 
 namespace vlb {
 namespace {
-
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef _Float16 v8h __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ v4f mma(const v8bf& a, const v8bf& b, const v4f& c)
-{
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
-__device__ __forceinline__ v4f mma(const v8h& a, const v8h& b, const v4f& c)
-{
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-}
-
-// 16-bit operand types of the projection forms.
-template <int PREC> struct Half;
-template <> struct Half<VLB_PREC_BF16> {
-    using V = v8bf;
-    using T = __bf16;
-    static constexpr bool split = true;
-    static constexpr float scale = 1.0f;
-};
-template <> struct Half<VLB_PREC_BF16_FAST> {
-    using V = v8bf;
-    using T = __bf16;
-    static constexpr bool split = false;
-    static constexpr float scale = 1.0f;
-};
-template <> struct Half<VLB_PREC_FP16> {
-    using V = v8h;
-    using T = _Float16;
-    static constexpr bool split = true;
-    static constexpr float scale = 1.0f / 16.0f;
-};
-template <> struct Half<VLB_PREC_FP16_FAST> {
-    using V = v8h;
-    using T = _Float16;
-    static constexpr bool split = false;
-    static constexpr float scale = 1.0f / 16.0f;
-};
-
-// Project 8 rays x 3 channels held by this lane onto the 16x16 tile C.
-// Split forms: x = hi + lo with hi = T(x), lo = T(x - hi); products
-// lo*Yhi + hi*Ylo + hi*Yhi, smallest first.
-template <int PREC>
-__device__ __forceinline__ void project16(const float* lx, const float* ly, const float* lz,
-                                          const void* ytab, size_t lo_off, size_t vidx, v4f* C)
-{
-    using H = Half<PREC>;
-    using V = typename H::V;
-    using T = typename H::T;
-    const V* yt = static_cast<const V*>(ytab);
-    const V yhi = yt[vidx];
-    const float* src[3] = {lx, ly, lz};
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        V ahi, alo;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            float x = src[c][e] * H::scale;
-            if (PREC == VLB_PREC_FP16 || PREC == VLB_PREC_FP16_FAST)
-                x = fminf(fmaxf(x, -65504.0f), 65504.0f);
-            const T h = (T)x;
-            ahi[e] = h;
-            if constexpr (H::split) alo[e] = (T)(x - (float)h);
-        }
-        if constexpr (H::split) {
-            const V ylo = yt[lo_off + vidx];
-            C[c] = mma(alo, yhi, C[c]);
-            C[c] = mma(ahi, ylo, C[c]);
-        }
-        C[c] = mma(ahi, yhi, C[c]);
-    }
-}
 
 // ----------------------------------------------------------- trace + shade
 // One ray per lane, persistent grid-stride over "trace tiles" (16 probes x
 // 32 rays = 512 rays).  Thread t of a tile traces for probe 16*pg + (l & 15)
 // the ray 32*ch + 8*(l >> 4) + e, with l = t >> 3 and e = t & 7, and stores
 // its radiance at rad[((tile*3 + c)*64 + l)*8 + e]: the 16x16 MFMA A-operand
-// order of the projection kernel, written fully coalesced.  A wave covers 8
-// neighbouring probes x 8 neighbouring directions (both Morton sorted).
-// Lane -> ray maps inside a 512-ray trace tile (t = thread in tile):
-//   MAP 0: a wave = 8 probes x 8 directions
-//   MAP 1: a wave = 16 probes x 4 directions
-//   MAP 2: a wave = 2 probes x 32 directions
-// The radiance slot (l, e) is the same for every map.
-__device__ __forceinline__ void tile_map(int map, int t, int& l, int& e)
-{
-    if (map == 1) {
-        const int p = t & 15, ray = t >> 4;
-        l = p + 16 * (ray >> 3);
-        e = ray & 7;
-    } else if (map == 2) {
-        const int ray = t & 31, p = t >> 5;
-        l = p + 16 * (ray >> 3);
-        e = ray & 7;
-    } else {
-        l = t >> 3;
-        e = t & 7;
-    }
-}
-
+// order of the projection kernel, written fully coalesced.  The lane map
+// (tile_map, a.map) sets how many neighbouring probes x directions (probes
+// Morton sorted, directions on a Hilbert curve) one wave covers.
 // OCC > 0 asks the compiler for OCC waves per SIMD (VGPR budget 512/OCC).
 template <bool RECORD, bool COUNT, int OCC>
 __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
@@ -149,9 +55,8 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
     const size_t xend = min(nitems, xbeg + per_xcd);
     const size_t stride = (size_t)(gridDim.x >> 3) * 256;
     for (size_t it = xbeg + (size_t)(blockIdx.x >> 3) * 256 + tid; it < xend; it += stride) {
-        const int tile = (int)(it >> 9);
-        int l, e;
-        tile_map(a.map, (int)(it & 511), l, e);
+        int tile, l, e;
+        tile_map(a.map, it, tile, l, e);
         const int t = l * 8 + e;
         const int pg = tile / a.nchunk;
         const int ch = tile - pg * a.nchunk;
@@ -187,117 +92,20 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
 }
 
 // ------------------------------------------------------------- projection
-// C[probe][coef] += sum_ray rad[probe][ray] * Yw[ray][coef] on MFMA.  A
-// workgroup (4 waves) owns 16 probes x one slice of RS rays; the waves take
-// its 32-ray chunks round-robin.  Lane l = 16*g + pl holds, per channel, the
-// radiance of probe pl for rays 8g..8g+7 of the chunk (A operand) and the
-// basis values Y_pl of the same rays (B operand).  FP32 uses 8 x
-// v_mfma_f32_16x16x4_f32 per chunk and channel; the 16-bit forms 1 or 3 x
-// v_mfma_f32_16x16x32_{bf16,f16}.  Chunk results are Kahan-summed in FP32
-// across chunks and waves into one (sum, compensation) partial per
-// (slice, probe, channel, coef).  HBM-bound: 12 B of radiance per ray.
+// Tile = (probe group pg, slice sl): chunks sl*nch .. sl*nch + nch - 1 of the
+// group's trace tiles (consecutive in `rad`), partials of slice sl.
 template <int PREC>
 __global__ __launch_bounds__(256) void k_project(BakeArgs a)
 {
     __shared__ float red[3 * 24 * 64];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int g = lane >> 4;
-    const int pl = lane & 15;
     const int nch = a.RS >> 5;
     for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const int pg = tile / a.nslices;
         const int sl = tile - pg * a.nslices;
-        v4f S[3], Cp[3];
-        for (int c = 0; c < 3; ++c) {
-            S[c] = v4f{0.f, 0.f, 0.f, 0.f};
-            Cp[c] = v4f{0.f, 0.f, 0.f, 0.f};
-        }
-        for (int k = wave; k < nch; k += 4) {
-            const int ch = sl * nch + k;                 // chunk within the probe group
-            const int ttile = pg * a.nchunk + ch;        // trace tile
-            // streamed once: non-temporal 16-byte loads
-            const v4f* src = reinterpret_cast<const v4f*>(a.rad + (size_t)ttile * 1536) + lane * 2;
-            float lx[8], ly[8], lz[8];
-            {
-                const v4f x0 = __builtin_nontemporal_load(src), x1 = __builtin_nontemporal_load(src + 1);
-                const v4f y0 = __builtin_nontemporal_load(src + 128), y1 = __builtin_nontemporal_load(src + 129);
-                const v4f z0 = __builtin_nontemporal_load(src + 256), z1 = __builtin_nontemporal_load(src + 257);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    lx[q] = x0[q];
-                    lx[4 + q] = x1[q];
-                    ly[q] = y0[q];
-                    ly[4 + q] = y1[q];
-                    lz[q] = z0[q];
-                    lz[4 + q] = z1[q];
-                }
-            }
-            const size_t vidx = (size_t)(ch * 4 + g) * 16 + pl;  // ray octet 4*ch + g, coef pl
-            v4f C[3];
-            for (int c = 0; c < 3; ++c) C[c] = v4f{0.f, 0.f, 0.f, 0.f};
-            if constexpr (PREC == VLB_PREC_FP32) {
-                const float4* yt = static_cast<const float4*>(a.ytab) + vidx * 2;
-                const float4 y0 = yt[0], y1 = yt[1];
-                const float y[8] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    C[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(lx[e], y[e], C[0], 0, 0, 0);
-                    C[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ly[e], y[e], C[1], 0, 0, 0);
-                    C[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(lz[e], y[e], C[2], 0, 0, 0);
-                }
-            } else {
-                project16<PREC>(lx, ly, lz, a.ytab, (size_t)a.Rpad * 2, vidx, C);
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const float y = C[c][i] - Cp[c][i];
-                    const float t = S[c][i] + y;
-                    Cp[c][i] = (t - S[c][i]) - y;
-                    S[c][i] = t;
-                }
-        }
-        // cross-wave Kahan combine through LDS, fixed order
-        if (wave > 0) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    red[((wave - 1) * 24 + c * 4 + i) * 64 + lane] = S[c][i];
-                    red[((wave - 1) * 24 + 12 + c * 4 + i) * 64 + lane] = Cp[c][i];
-                }
-        }
-        __syncthreads();
-        if (wave == 0) {
-            for (int w = 0; w < 3; ++w)
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const float s2 = red[(w * 24 + c * 4 + i) * 64 + lane];
-                        const float c2 = red[(w * 24 + 12 + c * 4 + i) * 64 + lane];
-                        float y = s2 - Cp[c][i];
-                        float t = S[c][i] + y;
-                        Cp[c][i] = (t - S[c][i]) - y;
-                        S[c][i] = t;
-                        y = -c2 - Cp[c][i];
-                        t = S[c][i] + y;
-                        Cp[c][i] = (t - S[c][i]) - y;
-                        S[c][i] = t;
-                    }
-            // D layout: lane holds rows 4g..4g+3 (probes) of column pl (coef)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const size_t row = (size_t)sl * a.Ppad + pg * 16 + 4 * g + i;
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    a.partials[(row * 3 + c) * 16 + pl] = make_float2(S[c][i], Cp[c][i]);
-            }
-        }
-        __syncthreads();
+        const int ch0 = sl * nch;
+        const float* rad0 = a.rad + ((size_t)pg * a.nchunk + ch0) * 1536;
+        float2* prow = a.partials + ((size_t)sl * a.Ppad + (size_t)pg * 16) * 48;
+        project_tile<PREC>(rad0, nch, ch0, a.ytab, (size_t)a.Rpad * 2, red, prow);
     }
 }
 

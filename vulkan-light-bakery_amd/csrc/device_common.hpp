@@ -53,10 +53,6 @@ struct ShadeParams {
 constexpr int kLdsDepth = 16;  // stack entries kept in LDS per lane
 
 // ---------------------------------------------------------------- helpers
-__device__ __forceinline__ float comp(const float3& v, int k)
-{
-    return k == 0 ? v.x : (k == 1 ? v.y : v.z);
-}
 __device__ __forceinline__ float dot3(const float3& a, const float3& b)
 {
     return a.x * b.x + a.y * b.y + a.z * b.z;
@@ -69,49 +65,59 @@ __device__ __forceinline__ float dot3(const float3& a, const float3& b)
 // accept when they share a sign.
 struct WtRay {
     float3 o;
-    int    kx, ky, kz;
-    float  Sx, Sy, Sz;
+    // axis permutation (kx, ky, kz) as select masks: kz = 0 -> r1, kz = 1 ->
+    // r2, kz = 2 -> neither; swp exchanges kx and ky (d[kz] < 0)
+    bool  r1, r2, swp;
+    float Sx, Sy, Sz;
 };
+
+// (v[kx], v[ky], v[kz]) with selects only (no indexed access: a runtime
+// component index would be lowered to divergent branches).
+__device__ __forceinline__ float3 wt_perm(const WtRay& w, const float3& v)
+{
+    const float a = w.r1 ? v.y : (w.r2 ? v.z : v.x);
+    const float b = w.r1 ? v.z : (w.r2 ? v.x : v.y);
+    const float c = w.r1 ? v.x : (w.r2 ? v.y : v.z);
+    return make_float3(w.swp ? b : a, w.swp ? a : b, c);
+}
 
 __device__ __forceinline__ WtRay wt_setup(const float3& o, const float3& d)
 {
     WtRay w;
     w.o = o;
-    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-    int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
-    int kx = kz + 1;
-    if (kx == 3) kx = 0;
-    int ky = kx + 1;
-    if (ky == 3) ky = 0;
-    float dkz = comp(d, kz);
-    if (dkz < 0.0f) {
-        int t = kx;
-        kx = ky;
-        ky = t;
-    }
-    w.kx = kx;
-    w.ky = ky;
-    w.kz = kz;
-    w.Sx = comp(d, kx) / dkz;
-    w.Sy = comp(d, ky) / dkz;
-    w.Sz = 1.0f / dkz;
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    const int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+    w.r1 = kz == 0;
+    w.r2 = kz == 1;
+    w.swp = false;
+    const float3 pd0 = wt_perm(w, d);
+    w.swp = pd0.z < 0.0f;
+    const float3 pd = wt_perm(w, d);
+    w.Sx = pd.x / pd.z;
+    w.Sy = pd.y / pd.z;
+    w.Sz = 1.0f / pd.z;
     return w;
 }
 
-__device__ __forceinline__ bool wt_hit(const WtRay& w, const float3& v0, const float3& v1,
-                                       const float3& v2, float tmin, float tmax, float& t_out,
-                                       float& u_out, float& v_out)
+// Edge-function part of the test.  On acceptance returns t (correctly
+// rounded T / det) with the numerators V, W and det kept for the
+// barycentrics, which are divided out once for the final hit only.
+struct WtHit {
+    float t, V, W, det;
+};
+
+__device__ __forceinline__ bool wt_test(const WtRay& w, const float3& v0, const float3& v1,
+                                        const float3& v2, float tmin, float tmax, WtHit& h)
 {
-    const float3 A = make_float3(v0.x - w.o.x, v0.y - w.o.y, v0.z - w.o.z);
-    const float3 B = make_float3(v1.x - w.o.x, v1.y - w.o.y, v1.z - w.o.z);
-    const float3 C = make_float3(v2.x - w.o.x, v2.y - w.o.y, v2.z - w.o.z);
-    const float Akz = comp(A, w.kz), Bkz = comp(B, w.kz), Ckz = comp(C, w.kz);
-    const float Ax = comp(A, w.kx) - w.Sx * Akz;
-    const float Ay = comp(A, w.ky) - w.Sy * Akz;
-    const float Bx = comp(B, w.kx) - w.Sx * Bkz;
-    const float By = comp(B, w.ky) - w.Sy * Bkz;
-    const float Cx = comp(C, w.kx) - w.Sx * Ckz;
-    const float Cy = comp(C, w.ky) - w.Sy * Ckz;
+    const float3 A = wt_perm(w, make_float3(v0.x - w.o.x, v0.y - w.o.y, v0.z - w.o.z));
+    const float3 B = wt_perm(w, make_float3(v1.x - w.o.x, v1.y - w.o.y, v1.z - w.o.z));
+    const float3 C = wt_perm(w, make_float3(v2.x - w.o.x, v2.y - w.o.y, v2.z - w.o.z));
+    const float Ax = A.x - w.Sx * A.z;
+    const float Ay = A.y - w.Sy * A.z;
+    const float Bx = B.x - w.Sx * B.z;
+    const float By = B.y - w.Sy * B.z;
+    const float Cx = C.x - w.Sx * C.z;
+    const float Cy = C.y - w.Sy * C.z;
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
@@ -129,15 +135,28 @@ __device__ __forceinline__ bool wt_hit(const WtRay& w, const float3& v0, const f
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     const float det = U + V + W;
     if (det == 0.0f) return false;
-    const float Az = w.Sz * Akz;
-    const float Bz = w.Sz * Bkz;
-    const float Cz = w.Sz * Ckz;
+    const float Az = w.Sz * A.z;
+    const float Bz = w.Sz * B.z;
+    const float Cz = w.Sz * C.z;
     const float T = U * Az + V * Bz + W * Cz;
     const float t = T / det;
     if (!(t >= tmin && t <= tmax)) return false;
-    t_out = t;
-    u_out = V / det;
-    v_out = W / det;
+    h.t = t;
+    h.V = V;
+    h.W = W;
+    h.det = det;
+    return true;
+}
+
+__device__ __forceinline__ bool wt_hit(const WtRay& w, const float3& v0, const float3& v1,
+                                       const float3& v2, float tmin, float tmax, float& t_out,
+                                       float& u_out, float& v_out)
+{
+    WtHit h;
+    if (!wt_test(w, v0, v1, v2, tmin, tmax, h)) return false;
+    t_out = h.t;
+    u_out = h.V / h.det;
+    v_out = h.W / h.det;
     return true;
 }
 
@@ -229,7 +248,7 @@ __device__ int trace_closest(const SceneView& sc, const float3& o, const float3&
     if (sc.ntris <= 0) return -1;
     const WtRay w = wt_setup(o, d);
     const SlabRay s = slab_setup(o, d, sc.maxabs);
-    float tbest = tmax, bu = 0.0f, bv = 0.0f;
+    float tbest = tmax, bV = 0.0f, bW = 0.0f, bdet = 1.0f;
     int best = -1;
     int sp = 0;
     uint32_t node = 0;
@@ -241,35 +260,27 @@ __device__ int trace_closest(const SceneView& sc, const float3& o, const float3&
         int c0 = n.d.x, c1 = n.d.y;
         bool h0 = e0 != __builtin_huge_valf();
         bool h1 = e1 != __builtin_huge_valf();
-        if (h0 && c0 < 0) {
-            float3 v0, v1, v2;
-            int prim;
-            load_tri(sc.tris, ~c0, v0, v1, v2, prim);
-            if (COUNT) ++cnt.tris;
-            float t, u, v;
-            if (wt_hit(w, v0, v1, v2, tmin, tbest, t, u, v) &&
-                (best < 0 || t < tbest || prim < best)) {
-                tbest = t;
-                best = prim;
-                bu = u;
-                bv = v;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int ck = k == 0 ? c0 : c1;
+            const bool hk = k == 0 ? h0 : h1;
+            if (hk && ck < 0) {
+                float3 v0, v1, v2;
+                int prim;
+                load_tri(sc.tris, ~ck, v0, v1, v2, prim);
+                if (COUNT) ++cnt.tris;
+                WtHit h;
+                // t <= tbest holds on acceptance; an equal t keeps the lower prim id
+                if (wt_test(w, v0, v1, v2, tmin, tbest, h) && (best < 0 || h.t < tbest || prim < best)) {
+                    tbest = h.t;
+                    best = prim;
+                    bV = h.V;
+                    bW = h.W;
+                    bdet = h.det;
+                }
+                if (k == 0) h0 = false;
+                else h1 = false;
             }
-            h0 = false;
-        }
-        if (h1 && c1 < 0) {
-            float3 v0, v1, v2;
-            int prim;
-            load_tri(sc.tris, ~c1, v0, v1, v2, prim);
-            if (COUNT) ++cnt.tris;
-            float t, u, v;
-            if (wt_hit(w, v0, v1, v2, tmin, tbest, t, u, v) &&
-                (best < 0 || t < tbest || prim < best)) {
-                tbest = t;
-                best = prim;
-                bu = u;
-                bv = v;
-            }
-            h1 = false;
         }
         if (h0 && h1) {
             const bool first0 = e0 <= e1;
@@ -286,8 +297,8 @@ __device__ int trace_closest(const SceneView& sc, const float3& o, const float3&
     }
     if (best >= 0) {
         t_out = tbest;
-        u_out = bu;
-        v_out = bv;
+        u_out = bV / bdet;
+        v_out = bW / bdet;
     }
     return best;
 }
@@ -315,8 +326,8 @@ __device__ bool trace_any(const SceneView& sc, const float3& o, const float3& d,
             int prim;
             load_tri(sc.tris, ~c0, v0, v1, v2, prim);
             if (COUNT) ++cnt.tris;
-            float t, u, v;
-            if (wt_hit(w, v0, v1, v2, tmin, tmax, t, u, v)) return true;
+            WtHit h;
+            if (wt_test(w, v0, v1, v2, tmin, tmax, h)) return true;
             h0 = false;
         }
         if (h1 && c1 < 0) {
@@ -324,8 +335,8 @@ __device__ bool trace_any(const SceneView& sc, const float3& o, const float3& d,
             int prim;
             load_tri(sc.tris, ~c1, v0, v1, v2, prim);
             if (COUNT) ++cnt.tris;
-            float t, u, v;
-            if (wt_hit(w, v0, v1, v2, tmin, tmax, t, u, v)) return true;
+            WtHit h;
+            if (wt_test(w, v0, v1, v2, tmin, tmax, h)) return true;
             h1 = false;
         }
         if (h0 && h1) {

@@ -124,8 +124,50 @@ struct Context {
     bool sky_pending = false;
     // user allocations (vlb_device_alloc)
     std::map<void*, DevBuf> user_bufs;
+    // autotuner decisions: (scene id, rays per probe) -> trace_variant
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> tuned;
     std::recursive_mutex mtx;
 };
+
+// ------------------------------------------------------- shared tables
+// Direction set of a bake in traversal order (Hilbert curve over the
+// octahedral map) with its weighted SH basis table: [Rpad/8][16][8] FP32, or
+// 16-bit [hi][lo] halves for the MFMA forms.  Rpad = nslices * RS.
+struct RayTable {
+    uint32_t R = 0, Rpad = 0, RS = 0;
+    int nslices = 0;
+    int K = 0, precision = 0;
+    double wref = 0.0;        // 4 pi / R
+    DevBuf dirs;              // float4[Rpad]: xyz, w = 1 valid / 0 padding
+    DevBuf dir_perm;          // int[Rpad]: sorted ray -> canonical ray index
+    DevBuf ytab;
+};
+
+// Probes in Morton order, padded to a multiple of 16 with copies of the last.
+struct ProbeSet {
+    int P = 0, Ppad = 0;
+    DevBuf probes;            // float4[Ppad]
+    DevBuf probe_perm;        // int[Ppad]: sorted -> original index, -1 padding
+};
+
+template <class T>
+DevBuf upload_vec(Context& ctx, const std::vector<T>& v)
+{
+    DevBuf b = dev_alloc(ctx, v.size() * sizeof(T));
+    VLB_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx.stream));
+    return b;
+}
+
+// tables.cpp
+void validate(const vlb_bake_params& p);
+uint32_t rays_of(const vlb_bake_params& p);
+uint32_t slice_rays(const vlb_bake_params& p, uint32_t R, int npg, int pgrid);
+void build_ray_table(Context& ctx, const vlb_bake_params& p, int K, uint32_t RS, RayTable& t);
+void build_probe_set(Context& ctx, const float* xyz, uint32_t P, ProbeSet& s);
+ShadeParams shade_params(const vlb_bake_params& p);
+size_t scene_spill_bytes(const Scene& s, uint32_t stride);
+std::vector<uint32_t> morton_order(const float* xyz, size_t n);
+std::vector<uint32_t> direction_order(const float* xyz, size_t n);
 
 // ------------------------------------------------------------ bake plan
 struct BakeArgs {
@@ -139,7 +181,13 @@ struct BakeArgs {
     int nchunk;               // 32-ray chunks per probe group (Rpad / 32)
     int ntrace;               // trace tiles = npg * nchunk
     float* rad;               // [ntrace][3][64][8] radiance, MFMA A order
-    int map;                  // lane -> ray map of k_trace (0..2)
+    int map;                  // lane -> ray map of k_trace / k_rays (0..3)
+    // wavefront pipeline (wave.hip)
+    int4* hits;               // [slot] prim, t, u, v (bits); -1 miss, -2 no ray
+    uint8_t* vis;             // [slot][nvis] shadow-ray occlusion
+    int nvis;                 // shadow rays per primary ray (lights)
+    int refill;               // idle lanes that trigger a refill (1..64)
+    unsigned long long* wctr; // 16 work counters (per XCD, primary + shadow)
     float tmin, tmax;
     float2* partials;         // [nslices][Ppad][3][16] (sum, compensation)
     uint32_t* spill;
@@ -172,8 +220,14 @@ struct Plan {
     MergeArgs margs{};
     int grid = 0;             // trace grid (persistent)
     int occ = 0;              // k_trace waves-per-SIMD variant (0, 6, 8)
+    uint32_t variant = 0;     // trace_variant in use
+    bool wave = false;        // wavefront pipeline (variants 13..24)
+    int wgrid = 0;            // k_rays grid
+    DevBuf hits, vis, wctr;
     int pgrid = 0;            // projection grid
-    DevBuf probes, dirs, ytab, rad, partials, spill, out, out64, probe_perm, dir_perm, counters;
+    RayTable rt;              // sorted directions + basis table
+    ProbeSet ps;              // probes in Morton order
+    DevBuf rad, partials, spill, out, out64, counters;
     bool timing = false;
     bool use_graph = false;
     hipGraphExec_t graph_exec = nullptr;
@@ -199,6 +253,9 @@ void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
 int  trace_grid_size(Context& ctx, int occ);
 void launch_trace(const BakeArgs& a, bool record, bool count, int occ, int grid, hipStream_t stream);
 void launch_project(const BakeArgs& a, int precision, int grid, hipStream_t stream);
+// wave.hip
+int  wave_grid_size(Context& ctx);
+void launch_wave(const BakeArgs& a, bool record, bool count, int grid, hipStream_t stream);
 void launch_merge(const MergeArgs& m, hipStream_t stream);
 void launch_trace_closest(const SceneView& sc, const float* org, const float* dir, int n,
                           float tmin, float tmax, int* prim, float* t, float* u, float* v,

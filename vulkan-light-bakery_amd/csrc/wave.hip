@@ -1,0 +1,505 @@
+// wave.hip -- wavefront form of the trace + shade stage.
+//
+// The fused k_trace (bake.hip) runs primary traversal, shading and the shadow
+// query back to back in one lane: a wave waits for its slowest lane in each
+// phase, and lanes whose primary ray missed sit idle during shadow rays.
+// Here the same work is three kernels:
+//   k_rays<false>  closest hit for every primary ray -> 16 B hit records
+//   k_rays<true>   shadow query for every (light, ray) pair that needs one
+//                  -> one visibility byte per pair
+//   k_shade        per ray: sky or shading, visibility combined in light
+//                  order -> radiance in the MFMA operand order of k_project
+// Both traversal kernels are persistent with dynamic ray fetch: when at
+// least `refill` lanes of a wave are idle, the wave hands them the next rays
+// of its XCD's work range (one atomic per chunk of rays), so lanes stay busy
+// until the range is drained.  Items that need no traversal (padding, shadow
+// pairs with N.L <= 0 or a missed primary) are retired at fetch time.
+//
+// Determinism: every decision (hit, N.L > 0, occlusion) is a function of the
+// ray alone, computed with the same operation sequence as the fused kernel
+// and the CPU oracle, so the results do not depend on which lane or wave
+// processed which ray.
+#include "vlb_internal.hpp"
+
+namespace vlb {
+namespace {
+
+// item (in the trace-tile space of bake.hip) -> radiance slot, probe, ray
+__device__ __forceinline__ void item_ray(const BakeArgs& a, size_t it, int& slot, int& probe,
+                                         int& r)
+{
+    int tile, l, e;
+    // same maps as k_trace
+    if (a.map == 3) {
+        const int i = (int)(it & 1023);
+        const int p = i >> 6, rr = i & 63, rc = rr & 31;
+        tile = (int)(it >> 10) * 2 + (rr >> 5);
+        l = p + 16 * (rc >> 3);
+        e = rc & 7;
+    } else {
+        tile = (int)(it >> 9);
+        const int t = (int)(it & 511);
+        if (a.map == 1) {
+            const int p = t & 15, ray = t >> 4;
+            l = p + 16 * (ray >> 3);
+            e = ray & 7;
+        } else if (a.map == 2) {
+            const int ray = t & 31, p = t >> 5;
+            l = p + 16 * (ray >> 3);
+            e = ray & 7;
+        } else {
+            l = t >> 3;
+            e = t & 7;
+        }
+    }
+    slot = tile * 512 + l * 8 + e;
+    const int pg = tile / a.nchunk;
+    const int ch = tile - pg * a.nchunk;
+    probe = pg * 16 + (l & 15);
+    r = ch * 32 + 8 * (l >> 4) + e;
+}
+
+__device__ __forceinline__ void slot_ray(const BakeArgs& a, int slot, int& probe, int& r)
+{
+    const int tile = slot >> 9, t = slot & 511;
+    const int l = t >> 3, e = t & 7;
+    const int pg = tile / a.nchunk;
+    const int ch = tile - pg * a.nchunk;
+    probe = pg * 16 + (l & 15);
+    r = ch * 32 + 8 * (l >> 4) + e;
+}
+
+// Geometry of a recorded primary hit (env_map.rchit:63-68): same operations
+// as shade_ray in device_common.hpp.
+struct HitGeom {
+    float3 P, N, alb;
+};
+
+__device__ __forceinline__ HitGeom hit_geom(const SceneView& sc, const float3& o, const float3& d,
+                                            int prim, float t, float u, float v)
+{
+    const float4 s0 = sc.shade[3 * prim + 0];
+    const float4 s1 = sc.shade[3 * prim + 1];
+    const float4 s2 = sc.shade[3 * prim + 2];
+    const float b0 = 1.0f - u - v;
+    HitGeom g;
+    g.P = make_float3(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+    float3 m;
+    m.x = s0.x * b0 + s1.x * u + s2.x * v;
+    m.y = s0.y * b0 + s1.y * u + s2.y * v;
+    m.z = s0.z * b0 + s1.z * u + s2.z * v;
+    const float ml = sqrtf(dot3(m, m));
+    g.N = ml > 0.0f ? make_float3(m.x / ml, m.y / ml, m.z / ml) : make_float3(0.f, 0.f, 0.f);
+    g.alb = make_float3(s0.w, s1.w, s2.w);
+    return g;
+}
+
+// The shadow ray of light j for a hit (LAMBERT: the light list with a
+// two-sided normal; REFERENCE: the single point light of env_map.rchit).
+struct ShadowRay {
+    float3 org, dir, E;
+    float dist, cosv;
+    bool needed;
+};
+
+__device__ __forceinline__ ShadowRay shadow_ray(const ShadeParams& sp, const HitGeom& g,
+                                                const float3& d, int j)
+{
+    ShadowRay s;
+    if (sp.mode == VLB_SHADE_REFERENCE) {
+        const vlb_ref_shading& r = sp.ref;
+        const float3 L = make_float3(r.light_position[0] - g.P.x, r.light_position[1] - g.P.y,
+                                     r.light_position[2] - g.P.z);
+        const float Llen = sqrtf(dot3(L, L));
+        const float3 Ln = Llen > 0.0f ? make_float3(L.x / Llen, L.y / Llen, L.z / Llen)
+                                      : make_float3(0.f, 0.f, 0.f);
+        const float sdn = dot3(Ln, g.N);
+        s.cosv = sdn > 0.0f ? sdn : 0.0f;
+        s.org = make_float3(g.P.x + r.shadow_bias * g.N.x, g.P.y + r.shadow_bias * g.N.y,
+                            g.P.z + r.shadow_bias * g.N.z);
+        s.dir = Ln;
+        s.dist = Llen;
+        s.E = make_float3(0.f, 0.f, 0.f);
+        s.needed = s.cosv != 0.0f;
+        return s;
+    }
+    float3 N = g.N;
+    if (dot3(N, d) > 0.0f) N = make_float3(-N.x, -N.y, -N.z);
+    const vlb_light& Lj = sp.lights[j];
+    if (Lj.type == VLB_LIGHT_POINT) {
+        const float3 Lv = make_float3(Lj.vec[0] - g.P.x, Lj.vec[1] - g.P.y, Lj.vec[2] - g.P.z);
+        s.dist = sqrtf(dot3(Lv, Lv));
+        s.dir = s.dist > 0.0f ? make_float3(Lv.x / s.dist, Lv.y / s.dist, Lv.z / s.dist)
+                              : make_float3(0.f, 0.f, 0.f);
+        const float d2 = s.dist * s.dist;
+        s.E = make_float3(Lj.rgb[0] / d2, Lj.rgb[1] / d2, Lj.rgb[2] / d2);
+    } else {
+        s.dir = make_float3(-Lj.vec[0], -Lj.vec[1], -Lj.vec[2]);
+        s.dist = sp.t_max;
+        s.E = make_float3(Lj.rgb[0], Lj.rgb[1], Lj.rgb[2]);
+    }
+    s.cosv = dot3(N, s.dir);
+    s.needed = s.cosv > 0.0f;
+    s.org = make_float3(g.P.x + sp.shadow_bias * N.x, g.P.y + sp.shadow_bias * N.y,
+                        g.P.z + sp.shadow_bias * N.z);
+    return s;
+}
+
+// ------------------------------------------------------------ traversal
+struct TravState {
+    WtRay w;
+    SlabRay s;
+    float tmin, tb;
+    int best;
+    float bV, bW, bdet;
+    uint32_t node;
+    int sp;
+};
+
+__device__ __forceinline__ void trav_init(TravState& T, const SceneView& sc, const float3& o,
+                                          const float3& d, float tmin, float tmax)
+{
+    T.w = wt_setup(o, d);
+    T.s = slab_setup(o, d, sc.maxabs);
+    T.tmin = tmin;
+    T.tb = tmax;
+    T.best = -1;
+    T.bV = 0.0f;
+    T.bW = 0.0f;
+    T.bdet = 1.0f;
+    T.node = 0;
+    T.sp = 0;
+}
+
+// One node visit.  Returns true when the ray is finished (ANY: on the first
+// accepted triangle, with T.best = 0).
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool trav_step(TravState& T, const SceneView& sc, const Stack& st,
+                                          TraceCount& cnt)
+{
+    const BNode n = sc.nodes[T.node];
+    if (COUNT) ++cnt.nodes;
+    const float e0 = slab(T.s, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, T.tmin, T.tb);
+    const float e1 = slab(T.s, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, T.tmin, T.tb);
+    const int c0 = n.d.x, c1 = n.d.y;
+    bool h0 = e0 != __builtin_huge_valf();
+    bool h1 = e1 != __builtin_huge_valf();
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int ck = k == 0 ? c0 : c1;
+        const bool hk = k == 0 ? h0 : h1;
+        if (hk && ck < 0) {
+            float3 v0, v1, v2;
+            int prim;
+            load_tri(sc.tris, ~ck, v0, v1, v2, prim);
+            if (COUNT) ++cnt.tris;
+            WtHit h;
+            if (ANY) {
+                if (wt_test(T.w, v0, v1, v2, T.tmin, T.tb, h)) {
+                    T.best = 0;
+                    return true;
+                }
+            } else if (wt_test(T.w, v0, v1, v2, T.tmin, T.tb, h) &&
+                       (T.best < 0 || h.t < T.tb || prim < T.best)) {
+                T.tb = h.t;
+                T.best = prim;
+                T.bV = h.V;
+                T.bW = h.W;
+                T.bdet = h.det;
+            }
+            if (k == 0) h0 = false;
+            else h1 = false;
+        }
+    }
+    if (h0 && h1) {
+        const bool first0 = e0 <= e1;
+        push(st, T.sp, (uint32_t)(first0 ? c1 : c0));
+        T.node = (uint32_t)(first0 ? c0 : c1);
+        return false;
+    }
+    if (h0) {
+        T.node = (uint32_t)c0;
+        return false;
+    }
+    if (h1) {
+        T.node = (uint32_t)c1;
+        return false;
+    }
+    if (T.sp == 0) return true;
+    T.node = pop(st, T.sp);
+    return false;
+}
+
+// Number of rays a wave takes from its XCD's counter at once.
+constexpr int kChunk = 256;
+
+// Sets up item `it` for this lane.  Returns false when the item needs no
+// traversal (its result is written here).
+template <bool ANY>
+__device__ __forceinline__ bool item_setup(const BakeArgs& a, size_t it, TravState& T, int& slot_out,
+                                           int& light_out)
+{
+    if (!ANY) {
+        int slot, probe, r;
+        item_ray(a, it, slot, probe, r);
+        slot_out = slot;
+        const float4 dd = a.dirs[r];
+        if (probe >= a.P || dd.w == 0.0f || a.sc.ntris <= 0) {
+            a.hits[slot] = make_int4(probe >= a.P || dd.w == 0.0f ? -2 : -1, 0, 0, 0);
+            return false;
+        }
+        const float4 pp = a.probes[probe];
+        trav_init(T, a.sc, make_float3(pp.x, pp.y, pp.z), make_float3(dd.x, dd.y, dd.z), a.tmin,
+                  a.tmax);
+        return true;
+    }
+    // shadow items are light-major: it = j * nslots + slot
+    const size_t nslots = (size_t)a.ntrace * 512;
+    const int j = (int)(it / nslots);
+    const int slot = (int)(it - (size_t)j * nslots);
+    slot_out = slot;
+    light_out = j;
+    const int4 h = a.hits[slot];
+    uint8_t* vis = a.vis + (size_t)slot * a.nvis + j;
+    if (h.x < 0) {
+        *vis = 0;
+        return false;
+    }
+    int probe, r;
+    slot_ray(a, slot, probe, r);
+    const float4 pp = a.probes[probe];
+    const float4 dd = a.dirs[r];
+    const float3 o = make_float3(pp.x, pp.y, pp.z), d = make_float3(dd.x, dd.y, dd.z);
+    const HitGeom g = hit_geom(a.sc, o, d, h.x, __int_as_float(h.y), __int_as_float(h.z),
+                               __int_as_float(h.w));
+    const ShadowRay s = shadow_ray(a.sp, g, d, j);
+    if (!s.needed) {
+        *vis = 0;
+        return false;
+    }
+    trav_init(T, a.sc, s.org, s.dir, 0.0f, s.dist);
+    return true;
+}
+
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigned long long* ctr)
+{
+    __shared__ uint32_t lds[kLdsDepth * 256];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    Stack st;
+    st.lds = &lds[tid];
+    st.spill = a.spill + (size_t)blockIdx.x * 256 + tid;
+    st.spill_stride = a.spill_stride;
+    TraceCount cnt;
+    uint32_t ntraced = 0;
+    // this XCD's share of the item space
+    const int xcd = blockIdx.x & 7;
+    const size_t per_xcd = (nitems + 7) >> 3;
+    const size_t xbeg = (size_t)xcd * per_xcd;
+    const size_t xend = min(nitems, xbeg + per_xcd);
+    size_t pnext = 0, pend = 0;  // wave's current chunk [pnext, pend)
+    bool drained = xbeg >= xend;
+
+    TravState T;
+    int slot = 0, light = 0;
+    bool act = false;
+    for (;;) {
+        // --- refill idle lanes (wave-uniform branch)
+        uint64_t idle = __ballot(!act);
+        if (!drained && (int)__popcll(idle) >= a.refill) {
+            while (idle != 0 && !drained) {
+                if (pnext >= pend) {
+                    unsigned long long base = 0;
+                    if (lane == 0) base = atomicAdd(&ctr[xcd], (unsigned long long)kChunk);
+                    base = __shfl(base, 0);
+                    pnext = xbeg + base;
+                    pend = min(xend, pnext + kChunk);
+                    if (pnext >= xend) {
+                        drained = true;
+                        break;
+                    }
+                }
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                const size_t avail = pend - pnext;
+                const bool mine = !act && rank < avail;
+                const size_t take = min((size_t)__popcll(idle), avail);
+                if (mine) {
+                    act = item_setup<ANY>(a, pnext + rank, T, slot, light);
+                    if (act && ANY) ++ntraced;
+                }
+                pnext += take;
+                idle = __ballot(!act);
+                if ((int)__popcll(idle) < a.refill) break;
+            }
+        }
+        if (__ballot(act) == 0) {
+            if (drained) break;
+            continue;
+        }
+        // --- one traversal step for active lanes
+        if (act) {
+            if (trav_step<ANY, COUNT>(T, a.sc, st, cnt)) {
+                if (ANY) {
+                    a.vis[(size_t)slot * a.nvis + light] = (uint8_t)(T.best == 0 ? 1 : 0);
+                } else if (T.best >= 0) {
+                    a.hits[slot] = make_int4(T.best, __float_as_int(T.tb), __float_as_int(T.bV / T.bdet),
+                                             __float_as_int(T.bW / T.bdet));
+                } else {
+                    a.hits[slot] = make_int4(-1, 0, 0, 0);
+                }
+                act = false;
+            }
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&a.counters[0], (unsigned long long)cnt.nodes);
+        atomicAdd(&a.counters[1], (unsigned long long)cnt.tris);
+        if (ANY) atomicAdd(&a.counters[2], (unsigned long long)ntraced);
+    }
+}
+
+// Per ray: sky on a miss, else the shading of env_map.rchit (REFERENCE) or
+// Lambert, with the recorded visibility; radiance in MFMA operand order.
+template <bool RECORD>
+__global__ __launch_bounds__(256) void k_shade(BakeArgs a)
+{
+    const size_t nslots = (size_t)a.ntrace * 512;
+    for (size_t slot = (size_t)blockIdx.x * 256 + threadIdx.x; slot < nslots;
+         slot += (size_t)gridDim.x * 256) {
+        const int4 h = a.hits[slot];
+        int probe, r;
+        slot_ray(a, (int)slot, probe, r);
+        float3 L = make_float3(0.f, 0.f, 0.f);
+        const bool ref = a.sp.mode == VLB_SHADE_REFERENCE;
+        if (h.x != -2) {
+            const float4 dd = a.dirs[r];
+            const float3 d = make_float3(dd.x, dd.y, dd.z);
+            if (h.x < 0) {
+                L = sky_radiance(a.sp.sky, d);
+                if (ref) {
+                    if (a.sp.ref.flags & VLB_REF_SRGB) {
+                        L.x = srgb(L.x);
+                        L.y = srgb(L.y);
+                        L.z = srgb(L.z);
+                    }
+                    if (a.sp.ref.flags & VLB_REF_UNORM8) {
+                        L.x = unorm8(L.x);
+                        L.y = unorm8(L.y);
+                        L.z = unorm8(L.z);
+                    }
+                }
+            } else {
+                const float4 pp = a.probes[probe];
+                const float3 o = make_float3(pp.x, pp.y, pp.z);
+                const HitGeom g = hit_geom(a.sc, o, d, h.x, __int_as_float(h.y), __int_as_float(h.z),
+                                           __int_as_float(h.w));
+                const uint8_t* vis = a.vis + slot * a.nvis;
+                if (ref) {
+                    const vlb_ref_shading& rs = a.sp.ref;
+                    const ShadowRay s = shadow_ray(a.sp, g, d, 0);
+                    const bool in_shadow = s.needed ? vis[0] != 0 : true;
+                    float diffuse = 0.0f, spec = 0.0f;
+                    if (!in_shadow) {
+                        diffuse = rs.c_diffuse * s.cosv;
+                        const float dnl = dot3(g.N, s.dir);
+                        const float3 R = make_float3(s.dir.x - 2.0f * dnl * g.N.x,
+                                                     s.dir.y - 2.0f * dnl * g.N.y,
+                                                     s.dir.z - 2.0f * dnl * g.N.z);
+                        const float rd = dot3(R, d);
+                        spec = rs.c_specular * powf(rd > 0.0f ? rd : 0.0f, rs.glossiness);
+                    }
+                    const bool zero = g.alb.x == 0.0f && g.alb.y == 0.0f && g.alb.z == 0.0f;
+                    L.x = (zero ? 1.0f : g.alb.x) * (rs.ambient + diffuse + spec);
+                    L.y = (zero ? 1.0f : g.alb.y) * (rs.ambient + diffuse + spec);
+                    L.z = (zero ? 1.0f : g.alb.z) * (rs.ambient + diffuse + spec);
+                    if (rs.flags & VLB_REF_SRGB) {
+                        L.x = srgb(L.x);
+                        L.y = srgb(L.y);
+                        L.z = srgb(L.z);
+                    }
+                    if (rs.flags & VLB_REF_UNORM8) {
+                        L.x = unorm8(L.x);
+                        L.y = unorm8(L.y);
+                        L.z = unorm8(L.z);
+                    }
+                } else {
+                    float3 acc = make_float3(0.f, 0.f, 0.f);
+                    for (int j = 0; j < a.sp.nlights; ++j) {
+                        const ShadowRay s = shadow_ray(a.sp, g, d, j);
+                        if (s.needed && vis[j] == 0) {
+                            acc.x = acc.x + s.E.x * s.cosv;
+                            acc.y = acc.y + s.E.y * s.cosv;
+                            acc.z = acc.z + s.E.z * s.cosv;
+                        }
+                    }
+                    const float kInvPi = 0.318309886183790671538f;
+                    L = make_float3(g.alb.x * acc.x * kInvPi, g.alb.y * acc.y * kInvPi,
+                                    g.alb.z * acc.z * kInvPi);
+                }
+            }
+            if (RECORD) {
+                const size_t ri = (size_t)a.probe_perm[probe] * a.R + a.dir_perm[r];
+                if (a.rad_out) {
+                    a.rad_out[3 * ri + 0] = L.x;
+                    a.rad_out[3 * ri + 1] = L.y;
+                    a.rad_out[3 * ri + 2] = L.z;
+                }
+                if (a.hit_out) a.hit_out[ri] = h.x;
+            }
+        }
+        const size_t tile = slot >> 9, t = slot & 511;
+        float* out = a.rad + tile * 1536 + t;
+        out[0] = L.x;
+        out[512] = L.y;
+        out[1024] = L.z;
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+template <class K>
+int occupancy_of(K kernel, int cus)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess || nb < 1) nb = 1;
+    return nb * cus;
+}
+
+}  // namespace
+
+int wave_grid_size(Context& ctx)
+{
+    const int g = std::max(occupancy_of(k_rays<false, false>, ctx.num_cus),
+                           occupancy_of(k_rays<true, false>, ctx.num_cus));
+    return (g + 7) & ~7;
+}
+
+void launch_wave(const BakeArgs& a, bool record, bool count, int grid, hipStream_t st)
+{
+    VLB_REQUIRE(grid % 8 == 0, VLB_ERR_INVALID, "wave grid must be a multiple of 8");
+    const size_t nslots = (size_t)a.ntrace * 512;
+    // work counters: [0..7] primary, [8..15] shadow (cleared by a kernel so
+    // the sequence can be captured in a graph)
+    k_zero_u64<<<1, 64, 0, st>>>(a.wctr, 16);
+    if (count) k_rays<false, true><<<grid, 256, 0, st>>>(a, nslots, a.wctr);
+    else k_rays<false, false><<<grid, 256, 0, st>>>(a, nslots, a.wctr);
+    VLB_HIP(hipGetLastError());
+    const size_t nshadow = nslots * (size_t)a.nvis;
+    if (a.nvis > 0) {
+        if (count) k_rays<true, true><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8);
+        else k_rays<true, false><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8);
+        VLB_HIP(hipGetLastError());
+    }
+    const int sgrid = (int)std::min<size_t>((nslots + 255) / 256, 8192);
+    if (record) k_shade<true><<<sgrid, 256, 0, st>>>(a);
+    else k_shade<false><<<sgrid, 256, 0, st>>>(a);
+    VLB_HIP(hipGetLastError());
+}
+
+}  // namespace vlb

@@ -120,6 +120,7 @@ def _load(path: str = LIB_PATH) -> C.CDLL:
         "vlb_plan_use_graph": (I32, [VP, I32]),
         "vlb_plan_count": (I32, [VP]),
         "vlb_plan_stats": (I32, [VP, P(BakeStats)]),
+        "vlb_plan_autotune": (I32, [VP, U32, P(U32)]),
         "vlb_plan_destroy": (I32, [VP]),
         "vlb_bake_radiance": (I32, [VP, U32, P(BakeParams), F, SZ, P(I32), SZ]),
         "vlb_skybox_sh": (I32, [VP, F, U32, U32, U32, F, SZ]),
@@ -352,6 +353,11 @@ class Plan:
 
     def count(self):
         self._chk(lib.vlb_plan_count(self.h), "plan_count")
+
+    def autotune(self, iters: int = 5) -> int:
+        v = C.c_uint32(0)
+        self._chk(lib.vlb_plan_autotune(self.h, iters, C.byref(v)), "plan_autotune")
+        return v.value
 
     def stats(self) -> dict:
         s = BakeStats()
