@@ -329,6 +329,283 @@ def bench_c3(args):
     return line
 
 
+# ------------------------------------------------------------ c4 / c5 data
+C4_SCENES = (  # (label, kind, seed, param0, param1): ~10k, ~100k, ~500k, ~1.1M triangles
+    ("room 11k", 0, 40, 0, 0),
+    ("room 99k", 0, 41, 64, 77),
+    ("terrain 499k", 1, 42, 256, 18),
+    ("terrain 1.15M", 1, 43, 256, 50),
+)
+
+
+def _region_probes(rng, scene, n):
+    """n probes uniformly in a random axis-aligned region of the scene bounds."""
+    import numpy as np
+
+    lo = np.asarray(scene["bounds_min"], np.float64)
+    hi = np.asarray(scene["bounds_max"], np.float64)
+    ext = hi - lo
+    size = ext * rng.uniform(0.25, 1.0, 3)
+    start = lo + rng.uniform(0.0, 1.0, 3) * (ext - size)
+    return (start + rng.uniform(0.0, 1.0, (n, 3)) * size).astype(np.float32)
+
+
+def _sun_params(vlb, probes, rays, prec, sun, lights=None):
+    return vlb.Params(probes=probes, rays=rays, sh_order=2, precision=prec,
+                      shade_mode=vlb.SHADE_LAMBERT, sun_dir=sun.tolist(),
+                      lights=lights if lights is not None else
+                      [(vlb.LIGHT_DIRECTIONAL, (-sun).tolist(), (3.0, 3.0, 3.0))],
+                      shadow_bias=2e-3)
+
+
+def _err_sample(vlb, orc, oscenes, items, nprobe=8, seed=0):
+    """Max rel. SH error vs the FP64 oracle over nprobe probes of each
+    (scene index, params, coefficients) item."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    worst = 0.0
+    errs = []
+    for si, prm, got in items:
+        P = prm.probes.shape[0]
+        sub = np.sort(rng.choice(P, size=min(nprobe, P), replace=False))
+        q = vlb.Params(**{**prm.__dict__, "probes": prm.probes[sub].copy(), "_keep": []})
+        q.precision = 0
+        ref = oscenes[si].bake(q, mode64=True, nthreads=min(16, os.cpu_count() or 1))["coeffs"]
+        e = rel_err(got[sub], ref)
+        errs.append(e)
+        worst = max(worst, float(e.max()))
+    allv = np.concatenate([e.ravel() for e in errs])
+    return {"max": worst, "p99": float(np.percentile(allv, 99)), "probes": int(allv.size // 3)}
+
+
+def bench_c4(args):
+    """Batched form: 64 uneven jobs over 4 resident scenes (11k .. 1.15M
+    triangles), probes per job log-uniform in [64, 32768], rays per probe in
+    {1024, 4096, 16384}, L2, BF16 projection with FP32 accumulation, all jobs
+    in one batch (one persistent launch per pass per round, hipGraph),
+    shapes chosen by the batch autotuner; against the plain FP32 baker
+    (one vlb_bake call per job)."""
+    import numpy as np
+
+    import vlb
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+
+    ctx = vlb.Context(0)
+    scenes, sids, infos = [], [], []
+    for label, kind, seed, p0, p1 in C4_SCENES:
+        sc = vlb.generate_scene(kind, seed, p0, p1)
+        scenes.append(sc)
+        sids.append(ctx.upload_scene_dict(sc))
+        infos.append(ctx.scene_info(sids[-1]))
+    rng = np.random.default_rng(4)
+    sun = vlb.sun_direction(rng)
+    spec = []
+    for j in range(64):
+        si = int(rng.integers(0, 4))
+        P = int(round(np.exp(rng.uniform(np.log(64), np.log(32768)))))
+        R = int(rng.choice([1024, 4096, 16384]))
+        spec.append((si, _region_probes(rng, scenes[si], P), R))
+    jobs16 = [(sids[si], _sun_params(vlb, pr, R, PREC["bf16"], sun)) for si, pr, R in spec]
+    P_all = sum(pr.shape[0] for _, pr, _ in spec)
+    rays_all = sum(pr.shape[0] * R for _, pr, R in spec)
+
+    b = ctx.batch(jobs16)
+    variant, chunk = b.autotune(2)
+    b.count()
+    cnt = b.info()
+    b.use_graph(True)
+    for _ in range(max(1, args.warmup)):
+        b.execute()
+    b.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.execute()
+    b.sync()
+    t1 = time.perf_counter()
+    info = b.info()
+    el = (t1 - t0) / args.steps
+    got = [b.download(j) for j in range(len(jobs16))]
+
+    # the plain FP32 baker: one synchronous vlb_bake per job, in order
+    jobs32 = [(sids[si], _sun_params(vlb, pr, R, PREC["fp32"], sun)) for si, pr, R in spec]
+    p0 = time.perf_counter()
+    for sid, prm in jobs32:
+        ctx.bake(sid, prm)
+    p1 = time.perf_counter()
+    plain_s = p1 - p0
+
+    # error vs FP64 on 8 probes of one job per scene (and the smallest job)
+    oscenes = [orc.Scene.from_dict(sc) for sc in scenes]
+    pick = {}
+    for j, (si, pr, R) in enumerate(spec):
+        pick.setdefault(si, j)
+    items = [(spec[j][0], jobs16[j][1], got[j]) for j in sorted(pick.values())]
+    err = _err_sample(vlb, orc, oscenes, items)
+
+    rays = cnt["primary_rays"] + cnt["shadow_rays"]
+    line = {"metric": METRIC, "value": round(P_all / el, 1), "unit": "probes/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": max(1, args.warmup), "ms_per_step": round(el * 1e3, 2),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "data": "synthetic (4 seeded rooms/terrains, 64 seeded jobs, Fibonacci rays)",
+            "config": {"workload": "c4 batched form", "jobs": len(spec), "probes": P_all,
+                       "primary_rays": rays_all,
+                       "scenes": [{"scene": C4_SCENES[i][0], "triangles": infos[i]["num_tris"],
+                                   "bvh_depth": infos[i]["depth"]} for i in range(4)],
+                       "rounds": info["rounds"], "trace_variant": variant, "chunk": chunk},
+            "grays_per_s": round(rays / el / 1e9, 3),
+            "device_ms_per_batch": round(info["total_ms"], 2),
+            "per_ray": {"inner_nodes": round(cnt["node_visits"] / rays, 2),
+                        "triangles": round(cnt["tri_tests"] / rays, 2)},
+            "plain_fp32_baker": {"probes_per_s": round(P_all / plain_s, 1), "seconds": round(plain_s, 3),
+                                 "calls": len(jobs32)},
+            "speedup_vs_plain_fp32": round(plain_s / el, 3),
+            "sh_rel_err_vs_fp64": err,
+            "device_bytes": info["device_bytes"]}
+    b.destroy()
+    ctx.close()
+    return line
+
+
+def bench_c5(args):
+    """Serving form: a city block (400 buildings, ~2M triangles) and the 4
+    scenes of the batched form stay resident; rebake requests arrive as a
+    Poisson process (--rate per second); each moves the sun or adds a point
+    light (intensity U[10, 500]) and rebakes 256..65536 probes (log-uniform)
+    of a random region at 4096 rays per probe.  The server (BF16, continuous
+    batching, HBM-budget admission) against the plain per-request FP32
+    baker (vlb_bake per request, in arrival order) on the same schedule."""
+    import threading
+
+    import numpy as np
+
+    import vlb
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+
+    ctx = vlb.Context(0)
+    specs = (("city 2M", 2, 5, 0, 0),) + C4_SCENES
+    scenes, sids, infos = [], [], []
+    for label, kind, seed, p0, p1 in specs:
+        sc = vlb.generate_scene(kind, seed, p0, p1)
+        scenes.append(sc)
+        sids.append(ctx.upload_scene_dict(sc))
+        infos.append(ctx.scene_info(sids[-1]))
+    rng = np.random.default_rng(5)
+    sun0 = vlb.sun_direction(rng)
+    n = args.requests
+    gaps = rng.exponential(1.0 / args.rate, n)
+    arrive = np.cumsum(gaps) - gaps[0]
+    reqs = []
+    for i in range(n):
+        si = int(rng.integers(0, len(scenes)))
+        P = int(round(np.exp(rng.uniform(np.log(256), np.log(65536)))))
+        pr = _region_probes(rng, scenes[si], P)
+        if rng.uniform() < 0.5:   # the sun moves
+            sun = sun0 + rng.normal(0.0, 0.2, 3)
+            sun[1] = abs(sun[1]) + 0.1
+            sun /= np.linalg.norm(sun)
+            lights = [(vlb.LIGHT_DIRECTIONAL, (-sun).tolist(), (3.0, 3.0, 3.0))]
+        else:                     # a point light is added
+            sun = sun0
+            pos = pr[int(rng.integers(0, P))] + rng.normal(0.0, 1.0, 3).astype(np.float32)
+            lights = [(vlb.LIGHT_DIRECTIONAL, (-sun).tolist(), (3.0, 3.0, 3.0)),
+                      (vlb.LIGHT_POINT, pos.tolist(), (float(rng.uniform(10, 500)),) * 3)]
+        reqs.append((si, pr, sun, lights))
+
+    def params(r, prec):
+        si, pr, sun, lights = r
+        return _sun_params(vlb, pr, 4096, prec, sun, lights)
+
+    # ---- the server
+    srv = vlb.Server(ctx, precision=PREC["bf16"], state_budget=args.state_budget << 20,
+                     step_rays=args.step_rays)
+    # warm-up: one small request per scene (direction tables, graph capture)
+    for si in range(len(scenes)):
+        w = (si, reqs[0][1][:256], sun0, [(vlb.LIGHT_DIRECTIONAL, (-sun0).tolist(), (3.0, 3.0, 3.0))])
+        srv.wait(srv.submit(sids[si], params(w, PREC["bf16"])))
+    ps16 = [params(r, PREC["bf16"]) for r in reqs]
+    ids = [None] * n
+    base = {}
+
+    def feeder():
+        t0 = time.perf_counter()
+        base["t0"] = t0
+        for i in range(n):
+            dt = t0 + arrive[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids[i] = srv.submit(sids[reqs[i][0]], ps16[i])
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    lat, outs, infos_r = [], [], []
+    for i in range(n):
+        while ids[i] is None:
+            time.sleep(0.0005)
+        c, inf = srv.wait(ids[i])
+        outs.append(c)
+        infos_r.append(inf)
+        lat.append(inf["done_ms"] - inf["submit_ms"])
+    th.join()
+    st = srv.stats()
+    first = min(i["submit_ms"] for i in infos_r)
+    last = max(i["done_ms"] for i in infos_r)
+    P_all = sum(r[1].shape[0] for r in reqs)
+    srv_span = (last - first) / 1e3
+    srv.destroy()
+
+    # ---- the plain per-request FP32 baker on the same arrival schedule
+    ps32 = [params(r, PREC["fp32"]) for r in reqs]
+    plat = []
+    q0 = time.perf_counter()
+    for i in range(n):
+        now = time.perf_counter() - q0
+        if now < arrive[i]:
+            time.sleep(arrive[i] - now)
+        ctx.bake(sids[reqs[i][0]], ps32[i])
+        plat.append((time.perf_counter() - q0 - arrive[i]) * 1e3)
+    plain_span = time.perf_counter() - q0
+
+    # error vs FP64 on 8 probes of a few requests
+    oscenes = {}
+    items = []
+    for i in range(0, n, max(1, n // 5)):
+        si = reqs[i][0]
+        if si not in oscenes:
+            oscenes[si] = orc.Scene.from_dict(scenes[si])
+        items.append((si, ps16[i], outs[i]))
+    err = _err_sample(vlb, orc, oscenes, items)
+
+    lat = np.asarray(lat)
+    plat = np.asarray(plat)
+    line = {"metric": METRIC, "value": round(P_all / srv_span, 1), "unit": "probes/s", "n_gpus": 1,
+            "steps": n, "warmup": len(scenes), "ms_per_step": round(float(np.mean(lat)), 2),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "data": "synthetic (seeded city + 4 rooms/terrains, Poisson arrivals, Fibonacci rays)",
+            "config": {"workload": "c5 serving form", "requests": n, "rate_per_s": args.rate,
+                       "probes": P_all, "rays_per_probe": 4096,
+                       "scenes": [{"scene": specs[i][0], "triangles": infos[i]["num_tris"],
+                                   "bvh_depth": infos[i]["depth"]} for i in range(len(specs))],
+                       "state_budget_mib": args.state_budget, "step_rays": args.step_rays},
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                           "p99": round(float(np.percentile(lat, 99)), 2),
+                           "max": round(float(lat.max()), 2)},
+            "server": {"steps": st["steps"], "busy_ms": round(st["busy_ms"], 1),
+                       "active_peak": st["active_peak"], "state_bytes_peak": st["state_bytes_peak"],
+                       "span_s": round(srv_span, 3)},
+            "plain_fp32_per_request": {"probes_per_s": round(P_all / plain_span, 1),
+                                       "p50_ms": round(float(np.percentile(plat, 50)), 2),
+                                       "p99_ms": round(float(np.percentile(plat, 99)), 2),
+                                       "span_s": round(plain_span, 3)},
+            "sh_rel_err_vs_fp64": err}
+    ctx.close()
+    return line
+
+
 def bench_copy(args):
     """Streaming copy of 1 GiB (k_copy): attainable HBM bandwidth, and the
     reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
@@ -355,7 +632,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=["c2", "c1", "c3", "copy", "sweep"])
+    ap.add_argument("--config", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "copy", "sweep"])
+    ap.add_argument("--rate", type=float, default=20.0, help="c5: request arrivals per second")
+    ap.add_argument("--requests", type=int, default=200, help="c5: number of requests")
+    ap.add_argument("--state-budget", type=int, default=256, help="c5: server state arena (MiB)")
+    ap.add_argument("--step-rays", type=int, default=1 << 25, help="c5: primary rays per step")
     ap.add_argument("--precision", default="fp32", choices=sorted(PREC))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="trace kernel variant (0 = default)")
@@ -365,8 +646,8 @@ def main():
               "build (probes shard trivially; see the design notes).  Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
-    line = {"c2": bench_c2, "c1": bench_c1, "c3": bench_c3, "copy": bench_copy,
-            "sweep": sweep_c2}[args.config](args)
+    line = {"c2": bench_c2, "c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
+            "copy": bench_copy, "sweep": sweep_c2}[args.config](args)
     print(json.dumps(line), flush=True)
 
 

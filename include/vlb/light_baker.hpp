@@ -89,7 +89,7 @@ public:
         check(vlb_probe_grid(lo, hi, c.data(), settings_.order, xyz.data(), step), "vlb_probe_grid");
         gridStep_ = {step[0], step[1], step[2]};
         std::vector<vec3> out(xyz.size() / 3);
-        std::memcpy(out.data(), xyz.data(), xyz.size() * sizeof(float));
+        for (size_t i = 0; i < out.size(); ++i) out[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
         return out;
     }
 

@@ -285,6 +285,124 @@ vlb_status  vlb_bake_radiance(vlb_ctx* ctx, uint32_t scene, const vlb_bake_param
                               float* radiance, size_t radiance_len,
                               int32_t* hit_prim, size_t hit_len);
 
+/* ---------------------------------------------------------------- batches */
+/* Many bakes over any of the context's resident scenes, run as one unit.
+ * Each job has the semantics of vlb_bake (LightBaker::bake,
+ * src/baker/light_baker.cpp:287-328, with per-probe reset); the reference
+ * itself bakes one scene per process.  The batch is cut into rounds of at
+ * most max_round_rays primary rays; a round is one persistent launch per
+ * pass (closest hit, shadow rays, shading, MFMA projection) over every job
+ * it holds, and vlb_batch_use_graph captures all rounds as one hipGraph.
+ * A job's coefficients do not depend on the other jobs of the batch.  All
+ * jobs must use the same precision; params->trace_variant is ignored (the
+ * batch has its own, below). */
+typedef struct vlb_batch vlb_batch;
+typedef struct vlb_batch_job {
+    uint32_t               scene;
+    uint32_t               reserved;
+    const vlb_bake_params* params;   /* rays_per_slice 0: whole direction */
+                                     /* set up to 16384 rays per slice    */
+} vlb_batch_job;
+typedef struct vlb_batch_options {
+    uint64_t max_round_rays;  /* 0 = 2^28; at most 2^30                   */
+    uint32_t trace_variant;   /* 13..24 wavefront lane map x refill; 0=20 */
+    uint32_t chunk;           /* rays per work fetch: 64..1024, divides   */
+                              /* 1024; 0 = 256                            */
+    uint32_t keep_f64;        /* also keep FP64 coefficients              */
+    uint32_t reserved[3];
+} vlb_batch_options;
+typedef struct vlb_batch_info {
+    uint64_t primary_rays;
+    uint64_t shadow_rays;     /* after vlb_batch_count                    */
+    uint64_t node_visits;     /* after vlb_batch_count                    */
+    uint64_t tri_tests;       /* after vlb_batch_count                    */
+    uint64_t projection_tiles;
+    uint64_t device_bytes;    /* job state + tables + round buffers       */
+    double   total_ms;        /* mean device time of the timed executes   */
+    uint32_t jobs, rounds, grid, trace_variant, chunk, timed_runs;
+} vlb_batch_info;
+vlb_status  vlb_batch_create(vlb_ctx* ctx, const vlb_batch_job* jobs, uint32_t njobs,
+                             const vlb_batch_options* opt /* may be NULL */,
+                             vlb_batch** out);
+/* Enqueues every round (asynchronous; each execute is timed on the device). */
+vlb_status  vlb_batch_execute(vlb_batch* b);
+vlb_status  vlb_batch_sync(vlb_batch* b);
+/* Capture execute() as one hipGraph and replay it from then on. */
+vlb_status  vlb_batch_use_graph(vlb_batch* b, int32_t on);
+/* Coefficients of job `job`: [P][K][3] in the job's probe order. */
+vlb_status  vlb_batch_download(vlb_batch* b, uint32_t job, float* out, size_t out_len);
+vlb_status  vlb_batch_download_f64(vlb_batch* b, uint32_t job, double* out, size_t out_len);
+/* Autotuner: times the lane maps x refill thresholds (trace_variant 13..24)
+ * x work-fetch chunks {256, 1024} on a sample of every job's tiles (about
+ * 16M primary rays), `iters` runs each, and keeps the fastest. */
+vlb_status  vlb_batch_autotune(vlb_batch* b, uint32_t iters, uint32_t* variant,
+                               uint32_t* chunk);
+/* One counting pass (node visits, triangle tests, shadow rays). */
+vlb_status  vlb_batch_count(vlb_batch* b);
+vlb_status  vlb_batch_info_get(vlb_batch* b, vlb_batch_info* out);
+vlb_status  vlb_batch_destroy(vlb_batch* b);
+
+/* ---------------------------------------------------------------- serving */
+/* Rebake requests over resident scenes, served by continuous batching (no
+ * counterpart in the reference, whose baker runs once per process).  A
+ * scheduler thread admits requests in arrival order while a request slot is
+ * free and their probe state (probes, partial sums, outputs) fits the
+ * state_budget arena, and runs steps of at most step_rays primary rays that
+ * hold a share of every admitted request's remaining work, two steps in
+ * flight.  Each request has the semantics of vlb_bake and its coefficients
+ * do not depend on what else is served.  Scenes must stay resident while
+ * requests on them are pending.  vlb_submit copies everything it needs. */
+typedef struct vlb_server vlb_server;
+typedef struct vlb_server_options {
+    uint64_t state_budget;    /* device bytes of admitted request state;  */
+                              /* 0 = 2 GiB (allocated once, up front)     */
+    uint64_t step_rays;       /* primary rays per step; 0 = 2^25          */
+    uint64_t result_bytes;    /* pinned host arena for finished results   */
+                              /* until vlb_wait; 0 = 256 MiB              */
+    uint64_t upload_bytes;    /* pinned staging for new requests' probes  */
+                              /* per step; 0 = 32 MiB                     */
+    uint32_t max_active;      /* requests on the device at once; 0 = 64   */
+    uint32_t max_lights;      /* shadow rays per primary ray; 0 = 4       */
+    int32_t  precision;       /* VLB_PREC_* of every request              */
+    uint32_t trace_variant;   /* 13..24; 0 = 20                           */
+    uint32_t chunk;           /* rays per work fetch; 0 = 256             */
+    uint32_t max_depth;       /* deepest scene BVH served; 0 = 64         */
+    uint32_t reserved[2];
+} vlb_server_options;
+typedef struct vlb_request_info {
+    double   submit_ms;       /* server clock: ms since vlb_server_create */
+    double   admit_ms;        /* state allocated on the device            */
+    double   start_ms;        /* first step holding its work launched     */
+    double   done_ms;         /* coefficients on the host                 */
+    uint64_t primary_rays;
+    uint32_t steps;           /* steps that traced part of the request    */
+    int32_t  status;
+} vlb_request_info;
+typedef struct vlb_server_stats {
+    uint64_t submitted, completed, steps, primary_rays, probes;
+    uint64_t state_bytes_peak;
+    double   busy_ms;         /* device time of all retired steps         */
+    double   clock_ms;        /* server clock now                         */
+    uint32_t active_peak;     /* most requests admitted at once           */
+    uint32_t queued;          /* waiting for admission now                */
+    int32_t  status;          /* VLB_OK unless the scheduler failed       */
+    uint32_t reserved;
+} vlb_server_stats;
+vlb_status  vlb_server_create(vlb_ctx* ctx, const vlb_server_options* opt /* may be NULL */,
+                              vlb_server** out);
+/* Queues a request (non-blocking); *request receives its id. */
+vlb_status  vlb_submit(vlb_server* s, uint32_t scene, const vlb_bake_params* p,
+                       uint64_t* request);
+/* *done = 1 once the request's coefficients are on the host. */
+vlb_status  vlb_poll(vlb_server* s, uint64_t request, int32_t* done);
+/* Blocks until the request is done, copies its [P][K][3] coefficients to
+ * out (may be NULL), fills *info (may be NULL) and forgets the request. */
+vlb_status  vlb_wait(vlb_server* s, uint64_t request, float* out, size_t out_len,
+                     vlb_request_info* info);
+vlb_status  vlb_server_stats_get(vlb_server* s, vlb_server_stats* out);
+/* Finishes every submitted request, then stops the scheduler. */
+vlb_status  vlb_server_destroy(vlb_server* s);
+
 /* --------------------------------------------------- SH of images / skies */
 /* Skybox SH of float cubemaps: the build's form of skybox_sh.comp
  * (shaders/skybox_sh.comp:23-42, host src/skybox_manager.cpp:107-130) for a

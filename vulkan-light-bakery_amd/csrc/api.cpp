@@ -150,33 +150,6 @@ void launch_trace_stage(Plan& pl, bool record, bool count, hipStream_t st)
     else launch_trace(pl.args, record, count, record || count ? 0 : pl.occ, pl.grid, st);
 }
 
-template <class F>
-vlb_status guard(Context* ctx, F&& f)
-{
-    try {
-        f();
-        return VLB_OK;
-    } catch (const Error& e) {
-        if (ctx) ctx->last_error = e.msg;
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        if (ctx) ctx->last_error = "host allocation failed";
-        return VLB_ERR_OOM;
-    } catch (const std::exception& e) {
-        if (ctx) ctx->last_error = e.what();
-        return VLB_ERR_INVALID;
-    } catch (...) {
-        if (ctx) ctx->last_error = "unknown error";
-        return VLB_ERR_INVALID;
-    }
-}
-
-Scene& find_scene(Context& ctx, uint32_t id)
-{
-    auto it = ctx.scenes.find(id);
-    if (it == ctx.scenes.end()) throw Error{VLB_ERR_NOT_FOUND, "unknown scene id " + std::to_string(id)};
-    return *it->second;
-}
 
 
 void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)

@@ -349,6 +349,73 @@ __global__ void k_depth(const int* __restrict__ iparent, const int* __restrict__
     atomicMax(depth, d);
 }
 
+// Depth of every inner node (root 0), for the BVH4 collapse.
+__global__ void k_inner_depth(const int* __restrict__ iparent, int nin, int* __restrict__ depth,
+                              int* __restrict__ even)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nin) return;
+    int d = 0, p = i;
+    while (p != 0 && d < 8192) {
+        p = iparent[p] >> 1;
+        ++d;
+    }
+    depth[i] = d;
+    even[i] = (d & 1) == 0 ? 1 : 0;
+}
+
+__device__ __forceinline__ void child_box(const BNode& a, int k, float b[6])
+{
+    if (k == 0) {
+        b[0] = a.a.x; b[1] = a.a.y; b[2] = a.a.z; b[3] = a.a.w; b[4] = a.b.x; b[5] = a.b.y;
+    } else {
+        b[0] = a.b.z; b[1] = a.b.w; b[2] = a.c.x; b[3] = a.c.y; b[4] = a.c.z; b[5] = a.c.w;
+    }
+}
+
+// BVH2 -> BVH4: every even-depth BVH2 node becomes a BVH4 node whose
+// children are its grandchildren through inner children, or the child
+// itself when that is a leaf (2 to 4 entries).  Boxes are copied, so the
+// BVH4 is exactly as conservative as the BVH2.
+__global__ void k_collapse(const BNode* __restrict__ n2, const int* __restrict__ depth,
+                           const int* __restrict__ idx4, int nin, BNode4* __restrict__ n4)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nin || (depth[i] & 1)) return;
+    float bx[4][6];
+    int ch[4];
+    int m = 0;
+    const BNode a = n2[i];
+    for (int k = 0; k < 2; ++k) {
+        const int c = k == 0 ? a.d.x : a.d.y;
+        if (c < 0) {
+            child_box(a, k, bx[m]);
+            ch[m++] = c;
+            continue;
+        }
+        const BNode g = n2[c];
+        for (int q = 0; q < 2; ++q) {
+            const int gc = q == 0 ? g.d.x : g.d.y;
+            child_box(g, q, bx[m]);
+            ch[m++] = gc < 0 ? gc : idx4[gc];
+        }
+    }
+    for (; m < 4; ++m) {
+        for (int t = 0; t < 6; ++t) bx[m][t] = 0.0f;
+        ch[m] = kEmpty;
+    }
+    BNode4 o;
+    o.lox = make_float4(bx[0][0], bx[1][0], bx[2][0], bx[3][0]);
+    o.loy = make_float4(bx[0][1], bx[1][1], bx[2][1], bx[3][1]);
+    o.loz = make_float4(bx[0][2], bx[1][2], bx[2][2], bx[3][2]);
+    o.hix = make_float4(bx[0][3], bx[1][3], bx[2][3], bx[3][3]);
+    o.hiy = make_float4(bx[0][4], bx[1][4], bx[2][4], bx[3][4]);
+    o.hiz = make_float4(bx[0][5], bx[1][5], bx[2][5], bx[3][5]);
+    o.child = make_int4(ch[0], ch[1], ch[2], ch[3]);
+    o.pad = make_int4(0, 0, 0, 0);
+    n4[idx4[i]] = o;
+}
+
 inline int nblk(int n, int b = 256) { return (n + b - 1) / b; }
 
 }  // namespace
@@ -469,6 +536,33 @@ void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
                                          dep.as<unsigned int>());
     }
     VLB_HIP(hipGetLastError());
+    // BVH4: even-depth BVH2 nodes, numbered by an exclusive scan
+    DevBuf idepth = dev_alloc(ctx, (size_t)nin * sizeof(int));
+    DevBuf even = dev_alloc(ctx, (size_t)nin * sizeof(int));
+    DevBuf idx4 = dev_alloc(ctx, (size_t)nin * sizeof(int));
+    if (n == 1) {
+        const int zero = 0, one = 1;
+        VLB_HIP(hipMemcpyAsync(idepth.p, &zero, sizeof zero, hipMemcpyHostToDevice, st));
+        VLB_HIP(hipMemcpyAsync(even.p, &one, sizeof one, hipMemcpyHostToDevice, st));
+    } else {
+        k_inner_depth<<<nblk(nin), 256, 0, st>>>(iparent.as<int>(), nin, idepth.as<int>(),
+                                                 even.as<int>());
+        VLB_HIP(hipGetLastError());
+    }
+    size_t scan_bytes = 0;
+    VLB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, even.as<int>(), idx4.as<int>(), nin, st));
+    DevBuf scan_tmp = dev_alloc(ctx, scan_bytes > 0 ? scan_bytes : 16);
+    VLB_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, even.as<int>(), idx4.as<int>(),
+                                             nin, st));
+    int last[2] = {0, 0};
+    VLB_HIP(hipMemcpyAsync(&last[0], idx4.as<int>() + (nin - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+    VLB_HIP(hipMemcpyAsync(&last[1], even.as<int>() + (nin - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+    VLB_HIP(hipStreamSynchronize(st));
+    const int n4 = last[0] + last[1];
+    s.nodes4 = dev_alloc(ctx, (size_t)n4 * sizeof(BNode4));
+    k_collapse<<<nblk(nin), 256, 0, st>>>(s.nodes.as<BNode>(), idepth.as<int>(), idx4.as<int>(), nin,
+                                          s.nodes4.as<BNode4>());
+    VLB_HIP(hipGetLastError());
     VLB_HIP(hipEventRecord(e1, st));
     float hb[16];
     unsigned int depth = 0;
@@ -481,6 +575,7 @@ void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
     (void)hipEventDestroy(e1);
     s.build_ms = ms;
     s.nnodes = nin;
+    s.nnodes4 = n4;
     s.depth = depth;
     for (int a = 0; a < 3; ++a) {
         s.bmin[a] = hb[6 + a];

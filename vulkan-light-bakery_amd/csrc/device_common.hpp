@@ -29,11 +29,31 @@ struct __align__(16) BNode {
     int4   d;  // child0 child1 - -
 };
 
+// The traversal kernels walk a 4-wide BVH collapsed from it: one BVH4 node
+// = 128 bytes, the boxes of up to four children as structure-of-arrays
+// (lo.x of child c in lox[c], ...) and their ids: >= 0 inner BVH4 node,
+// < 0 leaf (~ leaf-order triangle), kEmpty an unused slot.  The root is
+// node 0.  Half the dependent node fetches of the BVH2 per ray.
+struct __align__(16) BNode4 {
+    float4 lox, loy, loz;
+    float4 hix, hiy, hiz;
+    int4   child;
+    int4   pad;
+};
+constexpr int kEmpty = (int)0x80000000;
+
+// Stack entries a BVH4 traversal can hold for a BVH2 of `depth` inner
+// levels: at most 3 pushes per BVH4 level, ceil(depth / 2) levels.
+__host__ __device__ inline uint32_t stack_bound(uint32_t depth)
+{
+    return 3u * ((depth + 1u) / 2u) + 1u;
+}
+
 // Leaf-order triangle = 3 x float4: (v0, prim id bits) (v1, 0) (v2, 0).
 // Shading record per prim id = 3 x float4: (n0, albedo.r) (n1, .g) (n2, .b).
 
 struct SceneView {
-    const BNode*  nodes;
+    const BNode4* nodes;
     const float4* tris;
     const float4* shade;
     int           ntris;
@@ -238,8 +258,77 @@ __device__ __forceinline__ void load_tri(const float4* tris, int leaf, float3& v
     prim = __float_as_int(a.w);
 }
 
+// Node fetch: the 112 bytes in use (7 x 16 B).
+__device__ __forceinline__ BNode4 load_node(const BNode4* nodes, uint32_t i)
+{
+    const BNode4* p = nodes + i;
+    BNode4 n;
+    n.lox = p->lox;
+    n.loy = p->loy;
+    n.loz = p->loz;
+    n.hix = p->hix;
+    n.hiy = p->hiy;
+    n.hiz = p->hiz;
+    n.child = p->child;
+    return n;
+}
+
+// Entry distances of the four children (+inf: missed or empty slot).
+__device__ __forceinline__ void slab4(const SlabRay& s, const BNode4& n, float tmin, float tbest,
+                                      float e[4], int c[4])
+{
+    c[0] = n.child.x;
+    c[1] = n.child.y;
+    c[2] = n.child.z;
+    c[3] = n.child.w;
+    e[0] = slab(s, n.lox.x, n.loy.x, n.loz.x, n.hix.x, n.hiy.x, n.hiz.x, tmin, tbest);
+    e[1] = slab(s, n.lox.y, n.loy.y, n.loz.y, n.hix.y, n.hiy.y, n.hiz.y, tmin, tbest);
+    e[2] = slab(s, n.lox.z, n.loy.z, n.loz.z, n.hix.z, n.hiy.z, n.hiz.z, tmin, tbest);
+    e[3] = slab(s, n.lox.w, n.loy.w, n.loz.w, n.hix.w, n.hiy.w, n.hiz.w, tmin, tbest);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (c[k] == kEmpty) e[k] = __builtin_huge_valf();
+}
+
+__device__ __forceinline__ void cswap(float& ea, int& ca, float& eb, int& cb)
+{
+    const bool sw = eb < ea;
+    const float te = sw ? eb : ea;
+    const int tc = sw ? cb : ca;
+    eb = sw ? ea : eb;
+    cb = sw ? ca : cb;
+    ea = te;
+    ca = tc;
+}
+
+// After the leaf tests (their e set to +inf): sort the inner children near
+// first, push all hit ones but the nearest (farthest first) and continue at
+// the nearest, else pop.  Returns false when the stack is empty.
+__device__ __forceinline__ bool next_node(float e[4], int c[4], const Stack& st, int& sp,
+                                          uint32_t& node)
+{
+    cswap(e[0], c[0], e[1], c[1]);
+    cswap(e[2], c[2], e[3], c[3]);
+    cswap(e[0], c[0], e[2], c[2]);
+    cswap(e[1], c[1], e[3], c[3]);
+    cswap(e[1], c[1], e[2], c[2]);
+    const float inf = __builtin_huge_valf();
+    if (e[3] != inf) push(st, sp, (uint32_t)c[3]);
+    if (e[2] != inf) push(st, sp, (uint32_t)c[2]);
+    if (e[1] != inf) push(st, sp, (uint32_t)c[1]);
+    if (e[0] != inf) {
+        node = (uint32_t)c[0];
+        return true;
+    }
+    if (sp == 0) return false;
+    node = pop(st, sp);
+    return true;
+}
+
 // Closest hit: lexicographic minimum of (t, prim id) over accepted
-// triangles in [tmin, tmax].  Returns the prim id or -1.
+// triangles in [tmin, tmax].  Returns the prim id or -1.  Any BVH whose
+// boxes are conservative gives the same answer: every triangle that could
+// beat the current best is tested, and ties go to the lower prim id.
 template <bool COUNT>
 __device__ int trace_closest(const SceneView& sc, const float3& o, const float3& d, float tmin,
                              float tmax, float& t_out, float& u_out, float& v_out,
@@ -253,21 +342,17 @@ __device__ int trace_closest(const SceneView& sc, const float3& o, const float3&
     int sp = 0;
     uint32_t node = 0;
     for (;;) {
-        const BNode n = sc.nodes[node];
+        const BNode4 n = load_node(sc.nodes, node);
         if (COUNT) ++cnt.nodes;
-        float e0 = slab(s, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, tmin, tbest);
-        float e1 = slab(s, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, tmin, tbest);
-        int c0 = n.d.x, c1 = n.d.y;
-        bool h0 = e0 != __builtin_huge_valf();
-        bool h1 = e1 != __builtin_huge_valf();
+        float e[4];
+        int c[4];
+        slab4(s, n, tmin, tbest, e, c);
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int ck = k == 0 ? c0 : c1;
-            const bool hk = k == 0 ? h0 : h1;
-            if (hk && ck < 0) {
+        for (int k = 0; k < 4; ++k) {
+            if (e[k] != __builtin_huge_valf() && c[k] < 0) {
                 float3 v0, v1, v2;
                 int prim;
-                load_tri(sc.tris, ~ck, v0, v1, v2, prim);
+                load_tri(sc.tris, ~c[k], v0, v1, v2, prim);
                 if (COUNT) ++cnt.tris;
                 WtHit h;
                 // t <= tbest holds on acceptance; an equal t keeps the lower prim id
@@ -278,22 +363,10 @@ __device__ int trace_closest(const SceneView& sc, const float3& o, const float3&
                     bW = h.W;
                     bdet = h.det;
                 }
-                if (k == 0) h0 = false;
-                else h1 = false;
+                e[k] = __builtin_huge_valf();
             }
         }
-        if (h0 && h1) {
-            const bool first0 = e0 <= e1;
-            push(st, sp, (uint32_t)(first0 ? c1 : c0));
-            node = (uint32_t)(first0 ? c0 : c1);
-        } else if (h0) {
-            node = (uint32_t)c0;
-        } else if (h1) {
-            node = (uint32_t)c1;
-        } else {
-            if (sp == 0) break;
-            node = pop(st, sp);
-        }
+        if (!next_node(e, c, st, sp, node)) break;
     }
     if (best >= 0) {
         t_out = tbest;
@@ -314,43 +387,24 @@ __device__ bool trace_any(const SceneView& sc, const float3& o, const float3& d,
     int sp = 0;
     uint32_t node = 0;
     for (;;) {
-        const BNode n = sc.nodes[node];
+        const BNode4 n = load_node(sc.nodes, node);
         if (COUNT) ++cnt.nodes;
-        float e0 = slab(s, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, tmin, tmax);
-        float e1 = slab(s, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, tmin, tmax);
-        int c0 = n.d.x, c1 = n.d.y;
-        bool h0 = e0 != __builtin_huge_valf();
-        bool h1 = e1 != __builtin_huge_valf();
-        if (h0 && c0 < 0) {
-            float3 v0, v1, v2;
-            int prim;
-            load_tri(sc.tris, ~c0, v0, v1, v2, prim);
-            if (COUNT) ++cnt.tris;
-            WtHit h;
-            if (wt_test(w, v0, v1, v2, tmin, tmax, h)) return true;
-            h0 = false;
+        float e[4];
+        int c[4];
+        slab4(s, n, tmin, tmax, e, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (e[k] != __builtin_huge_valf() && c[k] < 0) {
+                float3 v0, v1, v2;
+                int prim;
+                load_tri(sc.tris, ~c[k], v0, v1, v2, prim);
+                if (COUNT) ++cnt.tris;
+                WtHit h;
+                if (wt_test(w, v0, v1, v2, tmin, tmax, h)) return true;
+                e[k] = __builtin_huge_valf();
+            }
         }
-        if (h1 && c1 < 0) {
-            float3 v0, v1, v2;
-            int prim;
-            load_tri(sc.tris, ~c1, v0, v1, v2, prim);
-            if (COUNT) ++cnt.tris;
-            WtHit h;
-            if (wt_test(w, v0, v1, v2, tmin, tmax, h)) return true;
-            h1 = false;
-        }
-        if (h0 && h1) {
-            const bool first0 = e0 <= e1;
-            push(st, sp, (uint32_t)(first0 ? c1 : c0));
-            node = (uint32_t)(first0 ? c0 : c1);
-        } else if (h0) {
-            node = (uint32_t)c0;
-        } else if (h1) {
-            node = (uint32_t)c1;
-        } else {
-            if (sp == 0) break;
-            node = pop(st, sp);
-        }
+        if (!next_node(e, c, st, sp, node)) break;
     }
     return false;
 }

@@ -1,5 +1,5 @@
 // project_common.hpp -- the SH projection of one probe tile on MFMA, shared
-// by the single-bake kernel (bake.hip) and the multi-job kernels (batch.hip).
+// by the single-bake kernel (bake.hip) and the multi-job kernels (multi.hip).
 //
 // C[probe][coef] += sum_ray rad[probe][ray] * Yw[ray][coef].  A workgroup (4
 // waves) owns 16 probes x one slice of nch 32-ray chunks; the waves take the

@@ -158,10 +158,18 @@ void validate(const vlb_bake_params& p)
 
 size_t scene_spill_bytes(const Scene& s, uint32_t stride)
 {
-    const uint32_t e = s.depth > (uint32_t)kLdsDepth ? s.depth - kLdsDepth + 1 : 0;
+    const uint32_t sb = stack_bound(s.depth);
+    const uint32_t e = sb > (uint32_t)kLdsDepth ? sb - kLdsDepth : 0;
     return std::max<size_t>(16, (size_t)e * stride * 4);
 }
 
+
+Scene& find_scene(Context& ctx, uint32_t id)
+{
+    auto it = ctx.scenes.find(id);
+    if (it == ctx.scenes.end()) throw Error{VLB_ERR_NOT_FOUND, "unknown scene id " + std::to_string(id)};
+    return *it->second;
+}
 
 uint32_t rays_of(const vlb_bake_params& p)
 {

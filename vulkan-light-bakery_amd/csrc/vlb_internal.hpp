@@ -85,17 +85,18 @@ DevBuf dev_alloc(Context& ctx, size_t bytes);
 struct Scene {
     uint32_t id = 0;
     int ntris = 0;
-    int nnodes = 0;
-    DevBuf nodes, tris, shade;
+    int nnodes = 0;           // BVH2 inner nodes (build product, exported)
+    int nnodes4 = 0;          // BVH4 nodes (traversed)
+    DevBuf nodes, nodes4, tris, shade;
     float maxabs = 0.0f;
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
     uint32_t depth = 0;
     double build_ms = 0.0;
-    size_t bytes() const { return nodes.bytes + tris.bytes + shade.bytes; }
+    size_t bytes() const { return nodes.bytes + nodes4.bytes + tris.bytes + shade.bytes; }
     SceneView view() const
     {
         SceneView v;
-        v.nodes = nodes.as<BNode>();
+        v.nodes = nodes4.as<BNode4>();
         v.tris = tris.as<float4>();
         v.shade = shade.as<float4>();
         v.ntris = ntris;
@@ -158,7 +159,30 @@ DevBuf upload_vec(Context& ctx, const std::vector<T>& v)
     return b;
 }
 
+// Runs f, turning exceptions into a status code and the context's last error.
+template <class F>
+inline vlb_status guard(Context* ctx, F&& f)
+{
+    try {
+        f();
+        return VLB_OK;
+    } catch (const Error& e) {
+        if (ctx) ctx->last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->last_error = "host allocation failed";
+        return VLB_ERR_OOM;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->last_error = e.what();
+        return VLB_ERR_INVALID;
+    } catch (...) {
+        if (ctx) ctx->last_error = "unknown error";
+        return VLB_ERR_INVALID;
+    }
+}
+
 // tables.cpp
+Scene& find_scene(Context& ctx, uint32_t id);
 void validate(const vlb_bake_params& p);
 uint32_t rays_of(const vlb_bake_params& p);
 uint32_t slice_rays(const vlb_bake_params& p, uint32_t R, int npg, int pgrid);
@@ -244,6 +268,67 @@ struct Plan {
     ~Plan();
 };
 
+// ------------------------------------------------------- multi-job work
+// One bake job of a batch (or one request of the server) as the multi-job
+// kernels see it.  Probe groups and slices tile the job exactly as in a
+// single bake: projection tile pt = pg * nslices + sl covers probes
+// 16pg..16pg+15 and rays sl*RS..sl*RS+RS-1 (nchunk 32-ray chunks per group).
+struct JobDesc {
+    SceneView sc;
+    ShadeParams sp;
+    const float4* probes;     // Ppad, Morton order
+    const int* probe_perm;    // sorted -> original probe (-1 padding)
+    const float4* dirs;       // Rpad, Hilbert order
+    const void* ytab;         // basis table of the job's ray table
+    float2* partials;         // [nslices][Ppad][3][16]
+    float* out;               // [P][K][3]
+    double* out64;            // [P][K][3] or null
+    double scale;             // 4 pi / R (x16 for the FP16 forms)
+    int P, Ppad, R, Rpad, RS, nslices, nchunk, K;
+    float tmin, tmax;
+    int nvis;                 // shadow rays per primary ray
+    int pad_;
+};
+
+// A run of projection tiles pt0 .. pt0+npt-1 of one job inside a round.  Its
+// trace tiles are tb .. tb + npt*(RS/32) - 1 of the round (tb even, so a
+// pair of trace tiles never straddles two runs), its projection tiles
+// pb .. pb+npt-1.
+struct Segment {
+    int job, pt0, npt, tb, pb;
+    int pad_[3];
+};
+// Jobs merged by a round: merge threads base .. base + Ppad*48 - 1 (base a
+// multiple of 256, so a workgroup never straddles two jobs).
+struct MergeItem {
+    int job, base;
+};
+// Counts of a round, read by the kernels (device memory, so one captured
+// graph serves rounds of any size up to the buffers' capacity).
+struct WorkHeader {
+    int nseg, ntrace, nproj, nmerge, merge_threads;
+    int nvis;                 // shadow rays per primary ray in this round (<= MultiArgs::nvis)
+    int pad_[2];
+};
+
+struct MultiArgs {
+    const JobDesc* jobs;
+    const WorkHeader* hdr;
+    const Segment* seg;
+    const MergeItem* merge;
+    float* rad;               // [ntrace][3][512] radiance, MFMA A order
+    int4* hits;               // [ntrace*512]
+    uint8_t* vis;             // [ntrace*512][nvis]
+    int nvis;                 // max shadow rays per primary ray over the jobs
+    int map;                  // lane -> ray map (0..3)
+    int refill;               // idle lanes that trigger a refill
+    int chunk;                // items per work fetch (64..1024, divides 1024)
+    unsigned long long* wctr; // 16 work counters: [0..7] primary, [8..15] shadow
+    uint32_t* spill;
+    uint32_t spill_stride;
+    unsigned long long* counters;  // node visits, tri tests, shadow rays (count mode)
+};
+
 // ----------------------------------------------------------- launchers
 // bvh.hip
 void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
@@ -263,6 +348,11 @@ void launch_trace_closest(const SceneView& sc, const float* org, const float* di
 void launch_trace_occluded(const SceneView& sc, const float* org, const float* dir,
                            const float* tmax, int n, float tmin, uint8_t* occ,
                            uint32_t* spill, uint32_t spill_stride, int grid, hipStream_t st);
+// multi.hip
+int  multi_grid_size(Context& ctx);
+void launch_multi_trace(const MultiArgs& a, bool count, int grid, int shade_grid, hipStream_t st);
+void launch_multi_project(const MultiArgs& a, int precision, int grid, hipStream_t st);
+void launch_multi_merge(const MultiArgs& a, int grid, hipStream_t st);
 // sh_image.hip
 void launch_skybox_sh(const float4* cubes, int N, int batch, int K, const float* wtab,
                       float* partial, int blocks_per_cube, float* out, hipStream_t st,

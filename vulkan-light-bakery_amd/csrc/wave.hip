@@ -19,7 +19,7 @@
 // ray alone, computed with the same operation sequence as the fused kernel
 // and the CPU oracle, so the results do not depend on which lane or wave
 // processed which ray.
-#include "vlb_internal.hpp"
+#include "wave_common.hpp"
 
 namespace vlb {
 namespace {
@@ -69,167 +69,6 @@ __device__ __forceinline__ void slot_ray(const BakeArgs& a, int slot, int& probe
     r = ch * 32 + 8 * (l >> 4) + e;
 }
 
-// Geometry of a recorded primary hit (env_map.rchit:63-68): same operations
-// as shade_ray in device_common.hpp.
-struct HitGeom {
-    float3 P, N, alb;
-};
-
-__device__ __forceinline__ HitGeom hit_geom(const SceneView& sc, const float3& o, const float3& d,
-                                            int prim, float t, float u, float v)
-{
-    const float4 s0 = sc.shade[3 * prim + 0];
-    const float4 s1 = sc.shade[3 * prim + 1];
-    const float4 s2 = sc.shade[3 * prim + 2];
-    const float b0 = 1.0f - u - v;
-    HitGeom g;
-    g.P = make_float3(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
-    float3 m;
-    m.x = s0.x * b0 + s1.x * u + s2.x * v;
-    m.y = s0.y * b0 + s1.y * u + s2.y * v;
-    m.z = s0.z * b0 + s1.z * u + s2.z * v;
-    const float ml = sqrtf(dot3(m, m));
-    g.N = ml > 0.0f ? make_float3(m.x / ml, m.y / ml, m.z / ml) : make_float3(0.f, 0.f, 0.f);
-    g.alb = make_float3(s0.w, s1.w, s2.w);
-    return g;
-}
-
-// The shadow ray of light j for a hit (LAMBERT: the light list with a
-// two-sided normal; REFERENCE: the single point light of env_map.rchit).
-struct ShadowRay {
-    float3 org, dir, E;
-    float dist, cosv;
-    bool needed;
-};
-
-__device__ __forceinline__ ShadowRay shadow_ray(const ShadeParams& sp, const HitGeom& g,
-                                                const float3& d, int j)
-{
-    ShadowRay s;
-    if (sp.mode == VLB_SHADE_REFERENCE) {
-        const vlb_ref_shading& r = sp.ref;
-        const float3 L = make_float3(r.light_position[0] - g.P.x, r.light_position[1] - g.P.y,
-                                     r.light_position[2] - g.P.z);
-        const float Llen = sqrtf(dot3(L, L));
-        const float3 Ln = Llen > 0.0f ? make_float3(L.x / Llen, L.y / Llen, L.z / Llen)
-                                      : make_float3(0.f, 0.f, 0.f);
-        const float sdn = dot3(Ln, g.N);
-        s.cosv = sdn > 0.0f ? sdn : 0.0f;
-        s.org = make_float3(g.P.x + r.shadow_bias * g.N.x, g.P.y + r.shadow_bias * g.N.y,
-                            g.P.z + r.shadow_bias * g.N.z);
-        s.dir = Ln;
-        s.dist = Llen;
-        s.E = make_float3(0.f, 0.f, 0.f);
-        s.needed = s.cosv != 0.0f;
-        return s;
-    }
-    float3 N = g.N;
-    if (dot3(N, d) > 0.0f) N = make_float3(-N.x, -N.y, -N.z);
-    const vlb_light& Lj = sp.lights[j];
-    if (Lj.type == VLB_LIGHT_POINT) {
-        const float3 Lv = make_float3(Lj.vec[0] - g.P.x, Lj.vec[1] - g.P.y, Lj.vec[2] - g.P.z);
-        s.dist = sqrtf(dot3(Lv, Lv));
-        s.dir = s.dist > 0.0f ? make_float3(Lv.x / s.dist, Lv.y / s.dist, Lv.z / s.dist)
-                              : make_float3(0.f, 0.f, 0.f);
-        const float d2 = s.dist * s.dist;
-        s.E = make_float3(Lj.rgb[0] / d2, Lj.rgb[1] / d2, Lj.rgb[2] / d2);
-    } else {
-        s.dir = make_float3(-Lj.vec[0], -Lj.vec[1], -Lj.vec[2]);
-        s.dist = sp.t_max;
-        s.E = make_float3(Lj.rgb[0], Lj.rgb[1], Lj.rgb[2]);
-    }
-    s.cosv = dot3(N, s.dir);
-    s.needed = s.cosv > 0.0f;
-    s.org = make_float3(g.P.x + sp.shadow_bias * N.x, g.P.y + sp.shadow_bias * N.y,
-                        g.P.z + sp.shadow_bias * N.z);
-    return s;
-}
-
-// ------------------------------------------------------------ traversal
-struct TravState {
-    WtRay w;
-    SlabRay s;
-    float tmin, tb;
-    int best;
-    float bV, bW, bdet;
-    uint32_t node;
-    int sp;
-};
-
-__device__ __forceinline__ void trav_init(TravState& T, const SceneView& sc, const float3& o,
-                                          const float3& d, float tmin, float tmax)
-{
-    T.w = wt_setup(o, d);
-    T.s = slab_setup(o, d, sc.maxabs);
-    T.tmin = tmin;
-    T.tb = tmax;
-    T.best = -1;
-    T.bV = 0.0f;
-    T.bW = 0.0f;
-    T.bdet = 1.0f;
-    T.node = 0;
-    T.sp = 0;
-}
-
-// One node visit.  Returns true when the ray is finished (ANY: on the first
-// accepted triangle, with T.best = 0).
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ bool trav_step(TravState& T, const SceneView& sc, const Stack& st,
-                                          TraceCount& cnt)
-{
-    const BNode n = sc.nodes[T.node];
-    if (COUNT) ++cnt.nodes;
-    const float e0 = slab(T.s, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, T.tmin, T.tb);
-    const float e1 = slab(T.s, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, T.tmin, T.tb);
-    const int c0 = n.d.x, c1 = n.d.y;
-    bool h0 = e0 != __builtin_huge_valf();
-    bool h1 = e1 != __builtin_huge_valf();
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int ck = k == 0 ? c0 : c1;
-        const bool hk = k == 0 ? h0 : h1;
-        if (hk && ck < 0) {
-            float3 v0, v1, v2;
-            int prim;
-            load_tri(sc.tris, ~ck, v0, v1, v2, prim);
-            if (COUNT) ++cnt.tris;
-            WtHit h;
-            if (ANY) {
-                if (wt_test(T.w, v0, v1, v2, T.tmin, T.tb, h)) {
-                    T.best = 0;
-                    return true;
-                }
-            } else if (wt_test(T.w, v0, v1, v2, T.tmin, T.tb, h) &&
-                       (T.best < 0 || h.t < T.tb || prim < T.best)) {
-                T.tb = h.t;
-                T.best = prim;
-                T.bV = h.V;
-                T.bW = h.W;
-                T.bdet = h.det;
-            }
-            if (k == 0) h0 = false;
-            else h1 = false;
-        }
-    }
-    if (h0 && h1) {
-        const bool first0 = e0 <= e1;
-        push(st, T.sp, (uint32_t)(first0 ? c1 : c0));
-        T.node = (uint32_t)(first0 ? c0 : c1);
-        return false;
-    }
-    if (h0) {
-        T.node = (uint32_t)c0;
-        return false;
-    }
-    if (h1) {
-        T.node = (uint32_t)c1;
-        return false;
-    }
-    if (T.sp == 0) return true;
-    T.node = pop(st, T.sp);
-    return false;
-}
-
 // Number of rays a wave takes from its XCD's counter at once.
 constexpr int kChunk = 256;
 
@@ -249,7 +88,7 @@ __device__ __forceinline__ bool item_setup(const BakeArgs& a, size_t it, TravSta
             return false;
         }
         const float4 pp = a.probes[probe];
-        trav_init(T, a.sc, make_float3(pp.x, pp.y, pp.z), make_float3(dd.x, dd.y, dd.z), a.tmin,
+        trav_init(T, a.sc.maxabs, make_float3(pp.x, pp.y, pp.z), make_float3(dd.x, dd.y, dd.z), a.tmin,
                   a.tmax);
         return true;
     }
@@ -270,14 +109,14 @@ __device__ __forceinline__ bool item_setup(const BakeArgs& a, size_t it, TravSta
     const float4 pp = a.probes[probe];
     const float4 dd = a.dirs[r];
     const float3 o = make_float3(pp.x, pp.y, pp.z), d = make_float3(dd.x, dd.y, dd.z);
-    const HitGeom g = hit_geom(a.sc, o, d, h.x, __int_as_float(h.y), __int_as_float(h.z),
+    const HitGeom g = hit_geom(a.sc.shade, o, d, h.x, __int_as_float(h.y), __int_as_float(h.z),
                                __int_as_float(h.w));
     const ShadowRay s = shadow_ray(a.sp, g, d, j);
     if (!s.needed) {
         *vis = 0;
         return false;
     }
-    trav_init(T, a.sc, s.org, s.dir, 0.0f, s.dist);
+    trav_init(T, a.sc.maxabs, s.org, s.dir, 0.0f, s.dist);
     return true;
 }
 
@@ -340,7 +179,7 @@ __global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigne
         }
         // --- one traversal step for active lanes
         if (act) {
-            if (trav_step<ANY, COUNT>(T, a.sc, st, cnt)) {
+            if (trav_step<ANY, COUNT>(T, a.sc.nodes, a.sc.tris, st, cnt)) {
                 if (ANY) {
                     a.vis[(size_t)slot * a.nvis + light] = (uint8_t)(T.best == 0 ? 1 : 0);
                 } else if (T.best >= 0) {
@@ -372,73 +211,10 @@ __global__ __launch_bounds__(256) void k_shade(BakeArgs a)
         int probe, r;
         slot_ray(a, (int)slot, probe, r);
         float3 L = make_float3(0.f, 0.f, 0.f);
-        const bool ref = a.sp.mode == VLB_SHADE_REFERENCE;
         if (h.x != -2) {
             const float4 dd = a.dirs[r];
             const float3 d = make_float3(dd.x, dd.y, dd.z);
-            if (h.x < 0) {
-                L = sky_radiance(a.sp.sky, d);
-                if (ref) {
-                    if (a.sp.ref.flags & VLB_REF_SRGB) {
-                        L.x = srgb(L.x);
-                        L.y = srgb(L.y);
-                        L.z = srgb(L.z);
-                    }
-                    if (a.sp.ref.flags & VLB_REF_UNORM8) {
-                        L.x = unorm8(L.x);
-                        L.y = unorm8(L.y);
-                        L.z = unorm8(L.z);
-                    }
-                }
-            } else {
-                const float4 pp = a.probes[probe];
-                const float3 o = make_float3(pp.x, pp.y, pp.z);
-                const HitGeom g = hit_geom(a.sc, o, d, h.x, __int_as_float(h.y), __int_as_float(h.z),
-                                           __int_as_float(h.w));
-                const uint8_t* vis = a.vis + slot * a.nvis;
-                if (ref) {
-                    const vlb_ref_shading& rs = a.sp.ref;
-                    const ShadowRay s = shadow_ray(a.sp, g, d, 0);
-                    const bool in_shadow = s.needed ? vis[0] != 0 : true;
-                    float diffuse = 0.0f, spec = 0.0f;
-                    if (!in_shadow) {
-                        diffuse = rs.c_diffuse * s.cosv;
-                        const float dnl = dot3(g.N, s.dir);
-                        const float3 R = make_float3(s.dir.x - 2.0f * dnl * g.N.x,
-                                                     s.dir.y - 2.0f * dnl * g.N.y,
-                                                     s.dir.z - 2.0f * dnl * g.N.z);
-                        const float rd = dot3(R, d);
-                        spec = rs.c_specular * powf(rd > 0.0f ? rd : 0.0f, rs.glossiness);
-                    }
-                    const bool zero = g.alb.x == 0.0f && g.alb.y == 0.0f && g.alb.z == 0.0f;
-                    L.x = (zero ? 1.0f : g.alb.x) * (rs.ambient + diffuse + spec);
-                    L.y = (zero ? 1.0f : g.alb.y) * (rs.ambient + diffuse + spec);
-                    L.z = (zero ? 1.0f : g.alb.z) * (rs.ambient + diffuse + spec);
-                    if (rs.flags & VLB_REF_SRGB) {
-                        L.x = srgb(L.x);
-                        L.y = srgb(L.y);
-                        L.z = srgb(L.z);
-                    }
-                    if (rs.flags & VLB_REF_UNORM8) {
-                        L.x = unorm8(L.x);
-                        L.y = unorm8(L.y);
-                        L.z = unorm8(L.z);
-                    }
-                } else {
-                    float3 acc = make_float3(0.f, 0.f, 0.f);
-                    for (int j = 0; j < a.sp.nlights; ++j) {
-                        const ShadowRay s = shadow_ray(a.sp, g, d, j);
-                        if (s.needed && vis[j] == 0) {
-                            acc.x = acc.x + s.E.x * s.cosv;
-                            acc.y = acc.y + s.E.y * s.cosv;
-                            acc.z = acc.z + s.E.z * s.cosv;
-                        }
-                    }
-                    const float kInvPi = 0.318309886183790671538f;
-                    L = make_float3(g.alb.x * acc.x * kInvPi, g.alb.y * acc.y * kInvPi,
-                                    g.alb.z * acc.z * kInvPi);
-                }
-            }
+            L = shade_record(a.sp, a.sc.shade, a.probes + probe, d, h, a.vis + slot * a.nvis);
             if (RECORD) {
                 const size_t ri = (size_t)a.probe_perm[probe] * a.R + a.dir_perm[r];
                 if (a.rad_out) {

@@ -84,6 +84,46 @@ class BakeStats(C.Structure):
                 ("slices", C.c_uint32), ("timed_runs", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+class BatchJob(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("reserved", C.c_uint32), ("params", C.POINTER(BakeParams))]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("max_round_rays", C.c_uint64), ("trace_variant", C.c_uint32),
+                ("chunk", C.c_uint32), ("keep_f64", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("primary_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
+                ("projection_tiles", C.c_uint64), ("device_bytes", C.c_uint64),
+                ("total_ms", C.c_double),
+                ("jobs", C.c_uint32), ("rounds", C.c_uint32), ("grid", C.c_uint32),
+                ("trace_variant", C.c_uint32), ("chunk", C.c_uint32), ("timed_runs", C.c_uint32)]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("state_budget", C.c_uint64), ("step_rays", C.c_uint64),
+                ("result_bytes", C.c_uint64), ("upload_bytes", C.c_uint64),
+                ("max_active", C.c_uint32), ("max_lights", C.c_uint32), ("precision", C.c_int32),
+                ("trace_variant", C.c_uint32), ("chunk", C.c_uint32), ("max_depth", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("submit_ms", C.c_double), ("admit_ms", C.c_double), ("start_ms", C.c_double),
+                ("done_ms", C.c_double), ("primary_rays", C.c_uint64), ("steps", C.c_uint32),
+                ("status", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("steps", C.c_uint64),
+                ("primary_rays", C.c_uint64), ("probes", C.c_uint64),
+                ("state_bytes_peak", C.c_uint64), ("busy_ms", C.c_double),
+                ("clock_ms", C.c_double), ("active_peak", C.c_uint32), ("queued", C.c_uint32),
+                ("status", C.c_int32), ("reserved", C.c_uint32)]
+
+
 class HostScene(C.Structure):
     _fields_ = [("nverts", C.c_uint32), ("ntris", C.c_uint32), ("xyz", C.POINTER(C.c_float)),
                 ("normals", C.POINTER(C.c_float)), ("idx", C.POINTER(C.c_uint32)),
@@ -123,6 +163,22 @@ def _load(path: str = LIB_PATH) -> C.CDLL:
         "vlb_plan_autotune": (I32, [VP, U32, P(U32)]),
         "vlb_plan_destroy": (I32, [VP]),
         "vlb_bake_radiance": (I32, [VP, U32, P(BakeParams), F, SZ, P(I32), SZ]),
+        "vlb_batch_create": (I32, [VP, P(BatchJob), U32, P(BatchOptions), P(VP)]),
+        "vlb_batch_execute": (I32, [VP]),
+        "vlb_batch_sync": (I32, [VP]),
+        "vlb_batch_use_graph": (I32, [VP, I32]),
+        "vlb_batch_download": (I32, [VP, U32, F, SZ]),
+        "vlb_batch_download_f64": (I32, [VP, U32, P(C.c_double), SZ]),
+        "vlb_batch_autotune": (I32, [VP, U32, P(U32), P(U32)]),
+        "vlb_batch_count": (I32, [VP]),
+        "vlb_batch_info_get": (I32, [VP, P(BatchInfo)]),
+        "vlb_batch_destroy": (I32, [VP]),
+        "vlb_server_create": (I32, [VP, P(ServerOptions), P(VP)]),
+        "vlb_submit": (I32, [VP, U32, P(BakeParams), P(U64)]),
+        "vlb_poll": (I32, [VP, U64, P(I32)]),
+        "vlb_wait": (I32, [VP, U64, F, SZ, P(RequestInfo)]),
+        "vlb_server_stats_get": (I32, [VP, P(ServerStats)]),
+        "vlb_server_destroy": (I32, [VP]),
         "vlb_skybox_sh": (I32, [VP, F, U32, U32, U32, F, SZ]),
         "vlb_skybox_sh_d": (I32, [VP, VP, U32, U32, U32, VP]),
         "vlb_skybox_last_ms": (I32, [VP, P(C.c_double), P(C.c_double)]),
@@ -376,6 +432,121 @@ class Plan:
             pass
 
 
+class Batch:
+    """Many bake jobs run as one unit (vlb_batch_*)."""
+
+    def __init__(self, ctx: "Context", handle: C.c_void_p, shapes):
+        self.ctx, self.h, self.shapes = ctx, handle, shapes
+
+    def _chk(self, st, what):
+        if st:
+            raise VlbError(st, f"{what}: {self.ctx.last_error()}")
+
+    def execute(self):
+        self._chk(lib.vlb_batch_execute(self.h), "batch_execute")
+
+    def sync(self):
+        self._chk(lib.vlb_batch_sync(self.h), "batch_sync")
+
+    def use_graph(self, on=True):
+        self._chk(lib.vlb_batch_use_graph(self.h, int(on)), "batch_use_graph")
+
+    def download(self, job: int) -> np.ndarray:
+        out = np.zeros(self.shapes[job], np.float32)
+        self._chk(lib.vlb_batch_download(self.h, job, _fptr(out), out.size), "batch_download")
+        return out
+
+    def download_f64(self, job: int) -> np.ndarray:
+        out = np.zeros(self.shapes[job], np.float64)
+        self._chk(lib.vlb_batch_download_f64(self.h, job, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                             out.size), "batch_download_f64")
+        return out
+
+    def autotune(self, iters: int = 3):
+        v, c = C.c_uint32(0), C.c_uint32(0)
+        self._chk(lib.vlb_batch_autotune(self.h, iters, C.byref(v), C.byref(c)), "batch_autotune")
+        return v.value, c.value
+
+    def count(self):
+        self._chk(lib.vlb_batch_count(self.h), "batch_count")
+
+    def info(self) -> dict:
+        s = BatchInfo()
+        self._chk(lib.vlb_batch_info_get(self.h, C.byref(s)), "batch_info")
+        return {k: getattr(s, k) for k, _ in BatchInfo._fields_}
+
+    def destroy(self):
+        if self.h:
+            lib.vlb_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Server:
+    """Rebake request server (vlb_server_*): continuous batching on the device."""
+
+    def __init__(self, ctx: "Context", precision: int = PREC_FP32, state_budget: int = 0,
+                 step_rays: int = 0, max_active: int = 0, max_lights: int = 0,
+                 trace_variant: int = 0, chunk: int = 0, result_bytes: int = 0,
+                 upload_bytes: int = 0):
+        self.ctx = ctx
+        o = ServerOptions()
+        o.state_budget, o.step_rays, o.max_active = state_budget, step_rays, max_active
+        o.max_lights, o.precision, o.trace_variant, o.chunk = max_lights, precision, trace_variant, chunk
+        o.result_bytes, o.upload_bytes = result_bytes, upload_bytes
+        self.h = C.c_void_p()
+        st = lib.vlb_server_create(ctx.h, C.byref(o), C.byref(self.h))
+        if st:
+            raise VlbError(st, f"server_create: {ctx.last_error()}")
+        self._shape = {}
+
+    def submit(self, sid: int, params: Params) -> int:
+        p = params.to_c()
+        rid = C.c_uint64(0)
+        st = lib.vlb_submit(self.h, sid, C.byref(p), C.byref(rid))
+        if st:
+            raise VlbError(st, f"submit: {self.ctx.last_error()}")
+        self._shape[rid.value] = (p.num_probes, params.K, 3)
+        return rid.value
+
+    def poll(self, rid: int) -> bool:
+        d = C.c_int32(0)
+        st = lib.vlb_poll(self.h, rid, C.byref(d))
+        if st:
+            raise VlbError(st, "poll")
+        return bool(d.value)
+
+    def wait(self, rid: int):
+        out = np.zeros(self._shape[rid], np.float32)
+        info = RequestInfo()
+        st = lib.vlb_wait(self.h, rid, _fptr(out), out.size, C.byref(info))
+        del self._shape[rid]
+        if st:
+            raise VlbError(st, f"wait: {self.ctx.last_error()}")
+        return out, {k: getattr(info, k) for k, _ in RequestInfo._fields_}
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        lib.vlb_server_stats_get(self.h, C.byref(s))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def destroy(self):
+        if self.h:
+            lib.vlb_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class Context:
     def __init__(self, device: int = 0, hbm_budget: int = 0):
         h = C.c_void_p()
@@ -482,6 +653,25 @@ class Context:
         h = C.c_void_p()
         self._chk(lib.vlb_plan_create(self.h, sid, C.byref(p), C.byref(h)), "plan_create")
         return Plan(self, h, p.num_probes, params.K)
+
+    def batch(self, jobs, max_round_rays: int = 0, trace_variant: int = 0, chunk: int = 0,
+              keep_f64: bool = False) -> Batch:
+        """jobs: sequence of (scene id, Params)."""
+        cps = [prm.to_c() for _, prm in jobs]
+        arr = (BatchJob * len(jobs))()
+        for i, ((sid, _), cp) in enumerate(zip(jobs, cps)):
+            arr[i].scene = sid
+            arr[i].params = C.pointer(cp)
+        opt = BatchOptions()
+        opt.max_round_rays = max_round_rays
+        opt.trace_variant = trace_variant
+        opt.chunk = chunk
+        opt.keep_f64 = int(keep_f64)
+        h = C.c_void_p()
+        self._chk(lib.vlb_batch_create(self.h, arr, len(jobs), C.byref(opt), C.byref(h)),
+                  "batch_create")
+        shapes = [(cp.num_probes, prm.K, 3) for (_, prm), cp in zip(jobs, cps)]
+        return Batch(self, h, shapes)
 
     def bake_radiance(self, sid: int, params: Params, want_radiance=True, want_hits=True):
         p = params.to_c()
