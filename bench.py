@@ -6,14 +6,15 @@ Default workload (the plain single-scene bake): procedural room 10x4x10 m
 4096 Fibonacci rays per probe, one directional light + procedural sky, one
 shadow ray per lit hit, L2 SH (9 coefficients x RGB), FP32 projection.
 
-A step is one full bake of that grid with every input resident in HBM: the
-fused trace/shade/project kernel plus the FP64 merge kernel (vlb_plan_execute).
-`value` is probes/s over the K timed steps (barrier + device sync on both
-sides).  The roofline figure is for the dominant kernel (k_bake), timed with
-hipEvents on the library's own stream inside the timed region.
+A step is one full bake of that grid with every input resident in HBM
+(vlb_plan_execute): the trace stage (closest-hit, shadow and shading kernels
+of the autotuned trace variant), the MFMA projection and the FP64 merge.
+`value` is probes/s over the K timed steps (device sync on both sides).  The
+roofline figure is for the trace stage, timed with hipEvents on the
+library's own stream inside the timed region.
 
     python bench.py [--steps K] [--warmup W] [--precision fp32|bf16|fp16]
-                    [--config c2|c1] [--no-cpu-baseline]
+                    [--config c2|c1|c3|c4|c5|copy|sweep] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -26,6 +27,8 @@ sys.path.insert(0, os.path.join(ROOT, "vulkan-light-bakery_amd"))
 
 METRIC = "probes/sec & Grays/s at ≤1e-3 rel SH err vs FP64; p50/p99 rebake latency"
 HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 112.0   # bytes fetched per BVH4 node visit (7 x 16 B)
+TRI_BYTES = 48.0     # bytes per triangle test (3 x float4)
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16_fast": 3, "fp16_fast": 4}
 
 
@@ -85,10 +88,10 @@ def bench_c2(args):
     rays_per_step = cnt["primary_rays"] + cnt["shadow_rays"]
     grays = rays_per_step * args.steps / elapsed / 1e9
 
-    # algorithmic bytes of one k_bake launch: 64 B per inner node fetched +
-    # 48 B per triangle tested (+ the 48 B shading record per primary hit is
-    # not counted), both counted by the kernel's counting variant
-    alg_bytes = 64.0 * cnt["node_visits"] + 48.0 * cnt["tri_tests"]
+    # algorithmic bytes of one trace stage: 112 B per BVH4 node fetched +
+    # 48 B per triangle tested (the 48 B shading record per primary hit is
+    # not counted), both counted by the kernels' counting variant
+    alg_bytes = NODE_BYTES * cnt["node_visits"] + TRI_BYTES * cnt["tri_tests"]
     trace_s = st["trace_ms"] / 1e3
     achieved = alg_bytes / trace_s / 1e9
     traffic = load_traffic("bake_c2_traffic.json")
@@ -310,7 +313,7 @@ def bench_c3(args):
             band["all"] = float(rel_err(got, ref).max())
             errs[f"L{order}_{name}"] = band
     rays = cnt["primary_rays"] + cnt["shadow_rays"]
-    alg = 64.0 * cnt["node_visits"] + 48.0 * cnt["tri_tests"]
+    alg = NODE_BYTES * cnt["node_visits"] + TRI_BYTES * cnt["tri_tests"]
     tr = runs["bf16"]["trace_ms"] / 1e3
     line = {"metric": METRIC, "value": runs["bf16"]["probes_per_s"], "unit": "probes/s", "n_gpus": 1,
             "steps": args.steps, "warmup": 1, "ms_per_step": runs["bf16"]["ms_per_step"],

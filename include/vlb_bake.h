@@ -431,6 +431,46 @@ vlb_status  vlb_project_equirect(vlb_ctx* ctx, const uint8_t* rgba8, uint32_t wi
 vlb_status  vlb_sh_reconstruct(vlb_ctx* ctx, const float* coeffs16x3, uint32_t width,
                                uint32_t height, uint8_t* out_rgba8);
 
+/* ------------------------------------------------------ probe-grid lookup */
+/* Consumer side of a baked grid: shaders/main.rchit:124-168 with
+ * shaders/sh.rmiss:23-39 -- for each surface point, the 8 probes of its grid
+ * cell, each visibility-tested with a shadow ray from the point (offset by
+ * `bias` along the normal) and weighted, and the SH of each evaluated at the
+ * point's normal.  The reference's lookup disagrees with its own producer in
+ * stride ((lmax+1)^2 = 289 vs 16 written), probe order (x + 7y + 49z vs the
+ * BFS order), grid (hard-coded 7^3, origin 0) and SH frame (normal unswizzled
+ * vs the bake's (x, z, y)); this form uses the grid as baked: `order`,
+ * `origin`, `count`, `num_coefs`, and the bake's SH frame.
+ *  REFERENCE  : weight = max(|step| - distance, 0) (main.rchit:141,155),
+ *               value = sum_{k<K} c_k Y_k(n) (sh.rmiss:30-37, SH radiance at
+ *               the normal).
+ *  IRRADIANCE : trilinear weights, value = sum_l (A_l / pi) sum_m c_lm Y_lm(n)
+ *               with A_0 = pi, A_1 = 2pi/3, A_2 = pi/4, A_3 = 0 (irradiance
+ *               / pi of the probe's radiance field).
+ * Occluded probes are dropped and the remaining weights renormalised; a point
+ * with no visible probe gets 0. */
+typedef struct vlb_probe_grid_desc {
+    float    origin[3];     /* position of probe (0, 0, 0)                   */
+    float    step[3];       /* gridStep                                      */
+    uint32_t count[3];      /* probes per axis                               */
+    int32_t  order;         /* VLB_ORDER_* of the coefficient buffer         */
+    uint32_t num_coefs;     /* K: 9 or 16                                    */
+    int32_t  sh_constants;  /* VLB_SH_* used to evaluate the basis           */
+} vlb_probe_grid_desc;
+enum { VLB_LOOKUP_REFERENCE = 0, VLB_LOOKUP_IRRADIANCE = 1 };
+typedef struct vlb_lookup_params {
+    int32_t  mode;          /* VLB_LOOKUP_*                                  */
+    uint32_t visibility;    /* 1: shadow ray to every probe                  */
+    float    bias;          /* ray origin offset along the normal            */
+    uint32_t reserved;
+} vlb_lookup_params;
+/* coeffs: [P][K][3] in the grid's order (host); points, normals: n x 3 host
+ * floats (normals unit length); out_rgb: n x 3. */
+vlb_status  vlb_probe_lookup(vlb_ctx* ctx, uint32_t scene, const vlb_probe_grid_desc* grid,
+                             const float* coeffs, const vlb_lookup_params* lp,
+                             const float* points, const float* normals, uint32_t n,
+                             float* out_rgb);
+
 /* ------------------------------------------------------- device buffers */
 vlb_status  vlb_device_alloc(vlb_ctx* ctx, size_t bytes, void** dptr);
 vlb_status  vlb_device_free(vlb_ctx* ctx, void* dptr);

@@ -116,6 +116,22 @@ void orc_equirect_project(const uint8_t* rgba8, uint32_t W, uint32_t H,
 void orc_sh_reconstruct(const float* coeffs16x3, uint32_t W, uint32_t H,
                         uint8_t* out);
 
+/* ------------------------------------------------------ probe-grid lookup */
+/* shaders/main.rchit:124-168 + shaders/sh.rmiss:23-39 in the fixed form of
+ * vlb_probe_lookup (grid as baked, bake's SH frame).  Visibility-ray geometry
+ * in float (same operations as the HIP kernel), SH and sums in double.
+ * s may be NULL (no visibility).  out: n x 3 doubles.  Returns 0, or -1 on
+ * bad arguments.
+ * Pinned by: partition of unity (equal probes -> that probe's value for any
+ * point and weights), trilinear reproduction of a field linear in position,
+ * irradiance of constant radiance L = L and of the l = 1 field L(d) = d.s ->
+ * (2/3) n.s (closed forms of the clamped-cosine convolution), BFS index
+ * table against orc_probe_grid's positions, and probes behind a wall
+ * dropped by the visibility rays. */
+int orc_probe_lookup(const orc_scene* s, const vlb_probe_grid_desc* g, const float* coeffs,
+                     const vlb_lookup_params* lp, const float* pts, const float* nrm, uint32_t n,
+                     double* out);
+
 /* ---------------------------------------------------------------- format */
 /* base64 of light_baker.cpp:330-373.  Pinned by Python's base64 (RFC 4648).
  * out must hold 4*ceil(n/3)+1 bytes; returns the encoded length. */

@@ -37,6 +37,8 @@ def _load():
         "orc_bake": (I32, [VP, C.POINTER(vlb.BakeParams), I32, I32, I32, D, F, C.POINTER(I32),
                            C.POINTER(C.c_uint64)]),
         "orc_sky": (None, [C.POINTER(vlb.Sky), D, D]),
+        "orc_probe_lookup": (I32, [VP, C.POINTER(vlb.ProbeGridDesc), F, C.POINTER(vlb.LookupParams),
+                                   F, F, U32, D]),
         "orc_skybox_sh": (None, [F, U32, I32, D]),
         "orc_equirect_project": (None, [C.POINTER(C.c_uint8), U32, U32, I32, D]),
         "orc_sh_reconstruct": (None, [F, U32, U32, C.POINTER(C.c_uint8)]),
@@ -155,6 +157,19 @@ class Scene:
                 self.h = None
         except Exception:
             pass
+
+
+def probe_lookup(scene, grid, coeffs, lp, points, normals) -> np.ndarray:
+    """scene: Scene or None (no visibility rays)."""
+    c = f32(coeffs)
+    pts = f32(points).reshape(-1, 3)
+    nrm = f32(normals).reshape(-1, 3)
+    out = np.zeros((pts.shape[0], 3), np.float64)
+    st = L.orc_probe_lookup(scene.h if scene is not None else None, C.byref(grid), _f(c), C.byref(lp),
+                            _f(pts), _f(nrm), pts.shape[0], _d(out))
+    if st:
+        raise ValueError("orc_probe_lookup rejected the arguments")
+    return out
 
 
 def skybox_sh(cube, K=9) -> np.ndarray:

@@ -874,6 +874,71 @@ vlb_status vlb_sh_reconstruct(vlb_ctx* h, const float* coeffs, uint32_t width, u
     });
 }
 
+// ------------------------------------------------------- probe-grid lookup
+vlb_status vlb_probe_lookup(vlb_ctx* h, uint32_t scene, const vlb_probe_grid_desc* g,
+                            const float* coeffs, const vlb_lookup_params* lp, const float* points,
+                            const float* normals, uint32_t n, float* out)
+{
+    if (!h || !g || !coeffs || !lp || (n && (!points || !normals || !out))) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_REQUIRE(g->num_coefs == 9 || g->num_coefs == 16, VLB_ERR_INVALID, "num_coefs must be 9 or 16");
+        VLB_REQUIRE(g->order == VLB_ORDER_LINEAR || g->order == VLB_ORDER_REFERENCE_BFS, VLB_ERR_INVALID,
+                    "bad probe order");
+        VLB_REQUIRE(lp->mode == VLB_LOOKUP_REFERENCE || lp->mode == VLB_LOOKUP_IRRADIANCE, VLB_ERR_INVALID,
+                    "bad lookup mode");
+        uint64_t P = 1;
+        for (int a = 0; a < 3; ++a) {
+            VLB_REQUIRE(g->count[a] >= 1, VLB_ERR_INVALID, "grid counts must be >= 1");
+            VLB_REQUIRE(g->step[a] != 0.0f && std::isfinite(g->step[a]), VLB_ERR_INVALID,
+                        "grid step must be finite and non-zero");
+            P *= g->count[a];
+        }
+        VLB_REQUIRE(P < (1ull << 28), VLB_ERR_INVALID, "grid too large");
+        VLB_REQUIRE(n < (1u << 28), VLB_ERR_INVALID, "too many lookup points");
+        if (n == 0) return;
+        Scene& s = find_scene(*c, scene);
+        VLB_HIP(hipSetDevice(c->device));
+        const size_t nc = (size_t)P * g->num_coefs * 3;
+        DevBuf d_c = dev_alloc(*c, nc * sizeof(float));
+        DevBuf d_p = dev_alloc(*c, (size_t)n * 3 * sizeof(float));
+        DevBuf d_n = dev_alloc(*c, (size_t)n * 3 * sizeof(float));
+        DevBuf d_o = dev_alloc(*c, (size_t)n * 3 * sizeof(float));
+        VLB_HIP(hipMemcpyAsync(d_c.p, coeffs, d_c.bytes, hipMemcpyHostToDevice, c->stream));
+        VLB_HIP(hipMemcpyAsync(d_p.p, points, d_p.bytes, hipMemcpyHostToDevice, c->stream));
+        VLB_HIP(hipMemcpyAsync(d_n.p, normals, d_n.bytes, hipMemcpyHostToDevice, c->stream));
+        const int grid = lookup_grid_size(*c);
+        const uint32_t stride = (uint32_t)grid * 256u;
+        DevBuf spill = dev_alloc(*c, scene_spill_bytes(s, stride));
+        LookupArgs a;
+        memset(&a, 0, sizeof a);
+        a.sc = s.view();
+        for (int ax = 0; ax < 3; ++ax) {
+            a.origin[ax] = g->origin[ax];
+            a.step[ax] = g->step[ax];
+            a.count[ax] = g->count[ax];
+        }
+        a.order = g->order;
+        a.K = (int)g->num_coefs;
+        a.exact = g->sh_constants == VLB_SH_EXACT;
+        a.mode = lp->mode;
+        a.vis = lp->visibility ? 1 : 0;
+        a.bias = lp->bias;
+        a.wmax = sqrtf(g->step[0] * g->step[0] + g->step[1] * g->step[1] + g->step[2] * g->step[2]);
+        a.coeffs = d_c.as<float>();
+        a.pts = d_p.as<float>();
+        a.nrm = d_n.as<float>();
+        a.n = (int)n;
+        a.out = d_o.as<float>();
+        a.spill = spill.as<uint32_t>();
+        a.spill_stride = stride;
+        launch_lookup(a, grid, c->stream);
+        VLB_HIP(hipMemcpyAsync(out, d_o.p, d_o.bytes, hipMemcpyDeviceToHost, c->stream));
+        VLB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
 // ---------------------------------------------------------- device buffers
 vlb_status vlb_device_alloc(vlb_ctx* h, size_t bytes, void** dptr)
 {

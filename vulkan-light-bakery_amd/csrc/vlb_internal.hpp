@@ -329,6 +329,22 @@ struct MultiArgs {
     unsigned long long* counters;  // node visits, tri tests, shadow rays (count mode)
 };
 
+// ------------------------------------------------------ probe-grid lookup
+struct LookupArgs {
+    SceneView sc;
+    float origin[3], step[3];
+    uint32_t count[3];
+    int order, K, exact, mode, vis;
+    float bias, wmax;
+    const float* coeffs;      // [P][K][3]
+    const float* pts;         // n x 3
+    const float* nrm;         // n x 3
+    int n;
+    float* out;               // n x 3
+    uint32_t* spill;
+    uint32_t spill_stride;
+};
+
 // ----------------------------------------------------------- launchers
 // bvh.hip
 void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
@@ -353,6 +369,9 @@ int  multi_grid_size(Context& ctx);
 void launch_multi_trace(const MultiArgs& a, bool count, int grid, int shade_grid, hipStream_t st);
 void launch_multi_project(const MultiArgs& a, int precision, int grid, hipStream_t st);
 void launch_multi_merge(const MultiArgs& a, int grid, hipStream_t st);
+// lookup.hip
+int  lookup_grid_size(Context& ctx);
+void launch_lookup(const LookupArgs& a, int grid, hipStream_t st);
 // sh_image.hip
 void launch_skybox_sh(const float4* cubes, int N, int batch, int K, const float* wtab,
                       float* partial, int blocks_per_cube, float* out, hipStream_t st,

@@ -33,6 +33,7 @@ SHADE_LAMBERT, SHADE_REFERENCE = 0, 1
 DIRS_FIBONACCI, DIRS_EQUIRECT = 0, 1
 SH_LITERAL, SH_EXACT = 0, 1
 ORDER_LINEAR, ORDER_REFERENCE_BFS = 0, 1
+LOOKUP_REFERENCE, LOOKUP_IRRADIANCE = 0, 1
 LIGHT_DIRECTIONAL, LIGHT_POINT = 0, 1
 REF_SRGB, REF_UNORM8 = 1, 2
 EQUIRECT_SH_COMP, EQUIRECT_SKYBOX_COMP = 0, 1
@@ -124,6 +125,33 @@ class ServerStats(C.Structure):
                 ("status", C.c_int32), ("reserved", C.c_uint32)]
 
 
+class ProbeGridDesc(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("step", C.c_float * 3), ("count", C.c_uint32 * 3),
+                ("order", C.c_int32), ("num_coefs", C.c_uint32), ("sh_constants", C.c_int32)]
+
+
+class LookupParams(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("visibility", C.c_uint32), ("bias", C.c_float),
+                ("reserved", C.c_uint32)]
+
+
+def grid_desc(origin, step, count, order=None, num_coefs=9, sh_constants=None) -> ProbeGridDesc:
+    g = ProbeGridDesc()
+    g.origin[:] = [float(x) for x in origin]
+    g.step[:] = [float(x) for x in step]
+    g.count[:] = [int(x) for x in count]
+    g.order = ORDER_LINEAR if order is None else order
+    g.num_coefs = num_coefs
+    g.sh_constants = SH_EXACT if sh_constants is None else sh_constants
+    return g
+
+
+def lookup_params(mode=0, visibility=True, bias=1e-3) -> LookupParams:
+    lp = LookupParams()
+    lp.mode, lp.visibility, lp.bias = mode, int(visibility), bias
+    return lp
+
+
 class HostScene(C.Structure):
     _fields_ = [("nverts", C.c_uint32), ("ntris", C.c_uint32), ("xyz", C.POINTER(C.c_float)),
                 ("normals", C.POINTER(C.c_float)), ("idx", C.POINTER(C.c_uint32)),
@@ -184,6 +212,7 @@ def _load(path: str = LIB_PATH) -> C.CDLL:
         "vlb_skybox_last_ms": (I32, [VP, P(C.c_double), P(C.c_double)]),
         "vlb_project_equirect": (I32, [VP, P(C.c_uint8), U32, U32, I32, F]),
         "vlb_sh_reconstruct": (I32, [VP, F, U32, U32, P(C.c_uint8)]),
+        "vlb_probe_lookup": (I32, [VP, U32, P(ProbeGridDesc), F, P(LookupParams), F, F, U32, F]),
         "vlb_device_alloc": (I32, [VP, SZ, P(VP)]),
         "vlb_device_free": (I32, [VP, VP]),
         "vlb_upload": (I32, [VP, VP, VP, SZ]),
@@ -684,6 +713,16 @@ class Context:
             None if hit is None else hit.ctypes.data_as(C.POINTER(C.c_int32)),
             0 if hit is None else hit.size), "bake_radiance")
         return rad, hit
+
+    def probe_lookup(self, sid: int, grid: "ProbeGridDesc", coeffs, lp: "LookupParams", points,
+                     normals) -> np.ndarray:
+        c = _f32(coeffs)
+        pts = _f32(points).reshape(-1, 3)
+        nrm = _f32(normals).reshape(-1, 3)
+        out = np.zeros((pts.shape[0], 3), np.float32)
+        self._chk(lib.vlb_probe_lookup(self.h, sid, C.byref(grid), _fptr(c), C.byref(lp), _fptr(pts),
+                                       _fptr(nrm), pts.shape[0], _fptr(out)), "probe_lookup")
+        return out
 
     # skies and images
     def skybox_sh(self, cubes: np.ndarray, sh_order: int = 2) -> np.ndarray:
