@@ -62,14 +62,18 @@ public:
         : name_(sceneName), settings_(std::move(s))
     {
         check(vlb_ctx_create(device, 0, &ctx_), "vlb_ctx_create");
-        load_scene(sceneName);
-        vlb_scene_info info{};
-        check(vlb_scene_info_get(ctx_, scene_, &info), "vlb_scene_info_get");
+        // scene bounds as Scene_t::getBounds (scene_manager.cpp:497-507):
+        // zero-initialised, so they always contain the origin
         std::array<vec3, 2> bounds;
-        bounds[0] = {std::min(0.0f, info.bounds_min[0]), std::min(0.0f, info.bounds_min[1]),
-                     std::min(0.0f, info.bounds_min[2])};
-        bounds[1] = {std::max(0.0f, info.bounds_max[0]), std::max(0.0f, info.bounds_max[1]),
-                     std::max(0.0f, info.bounds_max[2])};
+        if (!load_gltf(sceneName, bounds)) {
+            load_scene(sceneName);
+            vlb_scene_info info{};
+            check(vlb_scene_info_get(ctx_, scene_, &info), "vlb_scene_info_get");
+            bounds[0] = {std::min(0.0f, info.bounds_min[0]), std::min(0.0f, info.bounds_min[1]),
+                         std::min(0.0f, info.bounds_min[2])};
+            bounds[1] = {std::max(0.0f, info.bounds_max[0]), std::max(0.0f, info.bounds_max[1]),
+                         std::max(0.0f, info.bounds_max[2])};
+        }
         probePositions_ = probePositionsFromBoudingBox(bounds);
     }
     LightBaker(const LightBaker&) = delete;
@@ -151,9 +155,27 @@ private:
         }
     }
 
+    // LightBaker::LightBaker (light_baker.cpp:34-60): a .gltf / .glb asset
+    bool load_gltf(const std::string& name, std::array<vec3, 2>& bounds)
+    {
+        if (name.find(".gltf") == std::string::npos && name.find(".glb") == std::string::npos) return false;
+        vlb_host_scene* hs = nullptr;
+        char* json = nullptr;
+        check(vlb_gltf_load(name.c_str(), &hs, &json), ("vlb_gltf_load(" + name + ")").c_str());
+        vlb_status st = vlb_scene_upload(ctx_, hs->xyz, hs->nverts, hs->idx, hs->ntris, hs->normals,
+                                         hs->albedo, &scene_);
+        bounds[0] = {hs->bounds_min[0], hs->bounds_min[1], hs->bounds_min[2]};
+        bounds[1] = {hs->bounds_max[0], hs->bounds_max[1], hs->bounds_max[2]};
+        gltfText_ = json;
+        vlb_text_free(json);
+        vlb_host_scene_free(hs);
+        check(st, "vlb_scene_upload", ctx_);
+        return true;
+    }
+
     void load_scene(const std::string& name)
     {
-        // synthetic "<kind>[:<seed>]"; glTF ingest is not part of this path
+        // synthetic "<kind>[:<seed>]"
         std::string kind = name, seed_s = "0";
         const size_t colon = name.find(':');
         if (colon != std::string::npos) {
@@ -166,7 +188,7 @@ private:
         else if (kind == "city") k = VLB_SCENE_CITY;
         else if (kind == "cube") k = VLB_SCENE_CUBE;
         else if (kind == "spheres") k = VLB_SCENE_SPHERES;
-        else throw std::runtime_error("Select scene to bake: room|terrain|city|cube|spheres[:seed] (got '" + name + "')");
+        else throw std::runtime_error("Select scene to bake: a .gltf/.glb file or room|terrain|city|cube|spheres[:seed] (got '" + name + "')");
         vlb_host_scene* hs = nullptr;
         check(vlb_scene_generate(k, std::stoull(seed_s), 0, 0, &hs), "vlb_scene_generate");
         vlb_status st = vlb_scene_upload(ctx_, hs->xyz, hs->nverts, hs->idx, hs->ntris, hs->normals,

@@ -510,6 +510,14 @@ vlb_status  vlb_gltf_bake_json(const char* gltf_json, const float* coeffs,
                                uint32_t num_probes, uint32_t num_coefs,
                                const float grid_step[3], char* out, size_t out_len,
                                size_t* written);
+/* Reader of that file, Scene_t::loadBakedLight (scene_manager.cpp:613-648):
+ * light.gridStep, light.lmax (the coefficient count) and the coefficient
+ * floats of the buffer behind light.bufferView (its data URI decoded, the
+ * buffer's byteLength bytes kept).  coeffs may be NULL to query *num_floats.
+ * VLB_ERR_NOT_FOUND when the file has no "light". */
+vlb_status  vlb_gltf_read_bake(const char* gltf_json, float grid_step[3], uint32_t* num_coefs,
+                               float* coeffs, size_t coeffs_len, size_t* num_floats);
+
 
 /* -------------------------------------------------------- synthetic scenes */
 /* Seeded procedural scenes of the benchmark configurations.  kind:
@@ -533,6 +541,23 @@ typedef struct vlb_host_scene {
 } vlb_host_scene;
 enum { VLB_SCENE_ROOM = 0, VLB_SCENE_TERRAIN = 1, VLB_SCENE_CITY = 2,
        VLB_SCENE_CUBE = 3, VLB_SCENE_SPHERES = 4 };
+/* glTF scene ingest, Scene_t (src/scene_manager.cpp:31-66 file types,
+ * 241-337 vertex/index fetch, 445-538 node matrices and bounds, 837-857 the
+ * node tree of scenes[0]): .gltf (buffers as data URIs or files next to it)
+ * or .glb.  Every mesh primitive of every node becomes world-space triangles
+ * (node matrix T*R*S*M composed with its parents'); NORMAL (normalised at
+ * fetch, as the reference) is carried through the inverse transpose and left
+ * unnormalised (the shading normalises the interpolated normal, as
+ * env_map.rchit:63-68 does after its WorldToObject product); albedo is the
+ * material's baseColorFactor rgb, 1 when absent or all zero (textures are
+ * not sampled).  bounds_min/max follow the reference's scene bounds
+ * (scene_manager.cpp:497-507): they start at 0 and take each primitive's
+ * local min and max corners through its node's own matrix only.  *json_text
+ * (may be NULL) receives the glTF JSON (free with vlb_host_scene_free's
+ * companion vlb_text_free). */
+vlb_status  vlb_gltf_load(const char* path, vlb_host_scene** out, char** json_text);
+void        vlb_text_free(char* text);
+
 vlb_status  vlb_scene_generate(int32_t kind, uint64_t seed, uint32_t param0,
                                uint32_t param1, vlb_host_scene** out);
 void        vlb_host_scene_free(vlb_host_scene* s);

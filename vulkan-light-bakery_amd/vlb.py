@@ -225,6 +225,9 @@ def _load(path: str = LIB_PATH) -> C.CDLL:
         "vlb_gltf_bake_json": (I32, [C.c_char_p, F, U32, U32, F, C.c_char_p, SZ, P(SZ)]),
         "vlb_scene_generate": (I32, [I32, U64, U32, U32, P(P(HostScene))]),
         "vlb_host_scene_free": (None, [P(HostScene)]),
+        "vlb_gltf_load": (I32, [C.c_char_p, P(P(HostScene)), P(C.c_void_p)]),
+        "vlb_text_free": (None, [C.c_void_p]),
+        "vlb_gltf_read_bake": (I32, [C.c_char_p, F, P(U32), F, SZ, P(SZ)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -301,6 +304,49 @@ def gltf_bake_json(gltf_text: str, coeffs: np.ndarray, grid_step) -> str:
     if st:
         raise VlbError(st, "gltf_bake_json")
     return out.value.decode()
+
+
+def _host_scene_dict(ptr) -> dict:
+    s = ptr.contents
+    nv, nt = s.nverts, s.ntris
+    out = {
+        "xyz": np.ctypeslib.as_array(s.xyz, (max(1, nv * 3),))[:nv * 3].reshape(nv, 3).copy(),
+        "normals": np.ctypeslib.as_array(s.normals, (max(1, nv * 3),))[:nv * 3].reshape(nv, 3).copy(),
+        "idx": np.ctypeslib.as_array(s.idx, (max(1, nt * 3),))[:nt * 3].reshape(nt, 3).copy(),
+        "albedo": np.ctypeslib.as_array(s.albedo, (max(1, nt * 3),))[:nt * 3].reshape(nt, 3).copy(),
+        "bounds_min": np.array(s.bounds_min[:], np.float32),
+        "bounds_max": np.array(s.bounds_max[:], np.float32),
+    }
+    lib.vlb_host_scene_free(ptr)
+    return out
+
+
+def load_gltf(path: str):
+    """glTF / GLB scene -> (scene dict, glTF JSON text)."""
+    ptr = C.POINTER(HostScene)()
+    txt = C.c_void_p()
+    st = lib.vlb_gltf_load(str(path).encode(), C.byref(ptr), C.byref(txt))
+    if st:
+        raise VlbError(st, f"gltf_load({path})")
+    text = C.string_at(txt.value).decode() if txt.value else ""
+    lib.vlb_text_free(txt)
+    return _host_scene_dict(ptr), text
+
+
+def read_baked_gltf(text: str):
+    """Baked-light glTF -> (coefficients [n] float32, grid step, coefficient count)."""
+    step = np.zeros(3, np.float32)
+    k = C.c_uint32(0)
+    nf = C.c_size_t(0)
+    src = text.encode()
+    st = lib.vlb_gltf_read_bake(src, _fptr(step), C.byref(k), None, 0, C.byref(nf))
+    if st:
+        raise VlbError(st, "gltf_read_bake")
+    out = np.zeros(nf.value, np.float32)
+    st = lib.vlb_gltf_read_bake(src, _fptr(step), C.byref(k), _fptr(out), out.size, C.byref(nf))
+    if st:
+        raise VlbError(st, "gltf_read_bake")
+    return out, step, k.value
 
 
 def generate_scene(kind: int, seed: int, p0: int = 0, p1: int = 0) -> dict:
