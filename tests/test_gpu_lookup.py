@@ -2,8 +2,10 @@
 
 A grid is baked on the GPU, surface points come from rays traced into the
 room, and the lookup's value per point is compared with orc_probe_lookup on
-the same coefficients: relative error <= 1e-5 (FP32 vs double; a probe whose
-visibility differed would move the result by far more).
+the same coefficients: |got - ref| <= 1e-5 x max(|ref| of the point, median
+|ref|) -- FP32 against double, where the un-convolved radiance at a normal
+can nearly cancel; a probe whose visibility differed moves a point's value by
+its weight share times the probes' difference, far more than that.
 """
 import numpy as np
 import pytest
@@ -58,8 +60,10 @@ def test_lookup_matches_oracle(ctx, baked, order, mode, vis):
     lp = vlb.lookup_params(mode, vis, 2e-3)
     got = ctx.probe_lookup(sid, g, coeffs, lp, pts, nrm)
     ref = orc.probe_lookup(osc, g, coeffs, lp, pts, nrm)
-    err = np.abs(got - ref).max(axis=1) / np.maximum(np.abs(ref).max(axis=1), 1e-6)
-    assert err.max() <= 1e-5, f"max rel err {err.max():.3e}"
+    mag = np.abs(ref).max(axis=1)
+    err = np.abs(got - ref).max(axis=1) / np.maximum(mag, np.median(mag))
+    i = int(np.argmax(err))
+    assert err.max() <= 1e-5, f"max rel err {err.max():.3e} at {pts[i]} n={nrm[i]} got={got[i]} ref={ref[i]}"
     if vis:  # the walls and boxes do hide probes from some points
         lp0 = vlb.lookup_params(mode, False, 2e-3)
         assert not np.allclose(got, ctx.probe_lookup(sid, g, coeffs, lp0, pts, nrm))

@@ -2,8 +2,9 @@
 // 8-44): construct LightBaker, bake(), serialize(); exceptions map to
 // EXIT_FAILURE.  Extra options select the build's bake forms.
 //
-//   baker <room|terrain|city|cube|spheres>[:seed] [--grid N] [--rays R]
-//         [--lambert] [--order2] [--bf16|--fp16]
+//   baker <scene.gltf|scene.glb|room|terrain|city|cube|spheres[:seed]>
+//         [--grid N] [--rays R | --equirect W H] [--lambert] [--order2]
+//         [--bf16|--fp16]
 #include <chrono>
 #include <cstdlib>
 #include <iostream>
@@ -25,6 +26,10 @@ int main(int argc, char** argv)
             } else if (a == "--rays" && i + 1 < argc) {
                 s.dirMode = VLB_DIRS_FIBONACCI;
                 s.rays = (uint32_t)std::stoul(argv[++i]);
+            } else if (a == "--equirect" && i + 2 < argc) {
+                s.dirMode = VLB_DIRS_EQUIRECT;
+                s.equirectW = (uint32_t)std::stoul(argv[++i]);
+                s.equirectH = (uint32_t)std::stoul(argv[++i]);
             } else if (a == "--lambert") {
                 s.shadeMode = VLB_SHADE_LAMBERT;
                 s.shConstants = VLB_SH_EXACT;

@@ -203,7 +203,8 @@ __device__ __forceinline__ SlabRay slab_setup(const float3& o, const float3& d, 
     return s;
 }
 
-// returns the entry distance (for near-first ordering) or +inf on a miss
+// returns the (widened) entry distance, for near-first ordering, or +inf on
+// a miss
 __device__ __forceinline__ float slab(const SlabRay& s, float lox, float loy, float loz,
                                       float hix, float hiy, float hiz, float tmin, float tbest)
 {
@@ -213,8 +214,8 @@ __device__ __forceinline__ float slab(const SlabRay& s, float lox, float loy, fl
     float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
     float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
     float hi = fminf(tf, tbest);
-    hi = hi + fabsf(hi) * 4e-6f + s.apad;
-    float lo = tn - fabsf(tn) * 4e-6f - s.apad;
+    hi = fmaf(fabsf(hi), 4e-6f, hi) + s.apad;
+    const float lo = fmaf(-fabsf(tn), 4e-6f, tn) - s.apad;
     return (lo <= hi && hi >= tmin) ? lo : __builtin_huge_valf();
 }
 
@@ -271,6 +272,27 @@ __device__ __forceinline__ BNode4 load_node(const BNode4* nodes, uint32_t i)
     n.hiz = p->hiz;
     n.child = p->child;
     return n;
+}
+
+// Child c[k] of a 4-way node with a runtime k, through selects (a register
+// array indexed at run time would be placed in scratch memory).
+__device__ __forceinline__ int pick4(const int c[4], int k)
+{
+    return k == 0 ? c[0] : (k == 1 ? c[1] : (k == 2 ? c[2] : c[3]));
+}
+
+// Hit leaf children as a 4-bit mask; their entry distances are cleared so
+// that only inner children are sorted and pushed.
+__device__ __forceinline__ uint32_t take_leaves(float e[4], const int c[4])
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (e[k] != __builtin_huge_valf() && c[k] < 0) {
+            m |= 1u << k;
+            e[k] = __builtin_huge_valf();
+        }
+    return m;
 }
 
 // Entry distances of the four children (+inf: missed or empty slot).
@@ -347,23 +369,21 @@ __device__ int trace_closest(const SceneView& sc, const float3& o, const float3&
         float e[4];
         int c[4];
         slab4(s, n, tmin, tbest, e, c);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (e[k] != __builtin_huge_valf() && c[k] < 0) {
-                float3 v0, v1, v2;
-                int prim;
-                load_tri(sc.tris, ~c[k], v0, v1, v2, prim);
-                if (COUNT) ++cnt.tris;
-                WtHit h;
-                // t <= tbest holds on acceptance; an equal t keeps the lower prim id
-                if (wt_test(w, v0, v1, v2, tmin, tbest, h) && (best < 0 || h.t < tbest || prim < best)) {
-                    tbest = h.t;
-                    best = prim;
-                    bV = h.V;
-                    bW = h.W;
-                    bdet = h.det;
-                }
-                e[k] = __builtin_huge_valf();
+        // leaves one at a time per lane: the wave runs as many tests as its
+        // busiest lane has hit leaves, not one per child slot
+        for (uint32_t leaves = take_leaves(e, c); leaves != 0; leaves &= leaves - 1) {
+            float3 v0, v1, v2;
+            int prim;
+            load_tri(sc.tris, ~pick4(c, __builtin_ctz(leaves)), v0, v1, v2, prim);
+            if (COUNT) ++cnt.tris;
+            WtHit h;
+            // t <= tbest holds on acceptance; an equal t keeps the lower prim id
+            if (wt_test(w, v0, v1, v2, tmin, tbest, h) && (best < 0 || h.t < tbest || prim < best)) {
+                tbest = h.t;
+                best = prim;
+                bV = h.V;
+                bW = h.W;
+                bdet = h.det;
             }
         }
         if (!next_node(e, c, st, sp, node)) break;
@@ -392,17 +412,13 @@ __device__ bool trace_any(const SceneView& sc, const float3& o, const float3& d,
         float e[4];
         int c[4];
         slab4(s, n, tmin, tmax, e, c);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (e[k] != __builtin_huge_valf() && c[k] < 0) {
-                float3 v0, v1, v2;
-                int prim;
-                load_tri(sc.tris, ~c[k], v0, v1, v2, prim);
-                if (COUNT) ++cnt.tris;
-                WtHit h;
-                if (wt_test(w, v0, v1, v2, tmin, tmax, h)) return true;
-                e[k] = __builtin_huge_valf();
-            }
+        for (uint32_t leaves = take_leaves(e, c); leaves != 0; leaves &= leaves - 1) {
+            float3 v0, v1, v2;
+            int prim;
+            load_tri(sc.tris, ~pick4(c, __builtin_ctz(leaves)), v0, v1, v2, prim);
+            if (COUNT) ++cnt.tris;
+            WtHit h;
+            if (wt_test(w, v0, v1, v2, tmin, tmax, h)) return true;
         }
         if (!next_node(e, c, st, sp, node)) break;
     }

@@ -124,28 +124,26 @@ __device__ __forceinline__ bool trav_step(TravState& T, const BNode4* nodes, con
     float e[4];
     int c[4];
     slab4(T.s, n, T.tmin, T.tb, e, c);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (e[k] != __builtin_huge_valf() && c[k] < 0) {
-            float3 v0, v1, v2;
-            int prim;
-            load_tri(tris, ~c[k], v0, v1, v2, prim);
-            if (COUNT) ++cnt.tris;
-            WtHit h;
-            if (ANY) {
-                if (wt_test(T.w, v0, v1, v2, T.tmin, T.tb, h)) {
-                    T.best = 0;
-                    return true;
-                }
-            } else if (wt_test(T.w, v0, v1, v2, T.tmin, T.tb, h) &&
-                       (T.best < 0 || h.t < T.tb || prim < T.best)) {
-                T.tb = h.t;
-                T.best = prim;
-                T.bV = h.V;
-                T.bW = h.W;
-                T.bdet = h.det;
+    // leaves one at a time per lane (the wave runs as many tests as its
+    // busiest lane has hit leaves)
+    for (uint32_t leaves = take_leaves(e, c); leaves != 0; leaves &= leaves - 1) {
+        float3 v0, v1, v2;
+        int prim;
+        load_tri(tris, ~pick4(c, __builtin_ctz(leaves)), v0, v1, v2, prim);
+        if (COUNT) ++cnt.tris;
+        WtHit h;
+        if (ANY) {
+            if (wt_test(T.w, v0, v1, v2, T.tmin, T.tb, h)) {
+                T.best = 0;
+                return true;
             }
-            e[k] = __builtin_huge_valf();
+        } else if (wt_test(T.w, v0, v1, v2, T.tmin, T.tb, h) &&
+                   (T.best < 0 || h.t < T.tb || prim < T.best)) {
+            T.tb = h.t;
+            T.best = prim;
+            T.bV = h.V;
+            T.bW = h.W;
+            T.bdet = h.det;
         }
     }
     return !next_node(e, c, st, T.sp, T.node);
