@@ -30,6 +30,7 @@ HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 112.0   # bytes fetched per BVH4 node visit (7 x 16 B)
 TRI_BYTES = 48.0     # bytes per triangle test (3 x float4)
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16_fast": 3, "fp16_fast": 4}
+SCHED = {"srpt": 0, "share": 1, "fifo": 2}
 
 
 def rel_err(a, b):
@@ -525,7 +526,7 @@ def bench_c5(args):
 
     # ---- the server
     srv = vlb.Server(ctx, precision=PREC["bf16"], state_budget=args.state_budget << 20,
-                     step_rays=args.step_rays)
+                     step_rays=args.step_rays, policy=SCHED[args.policy])
     # warm-up: one small request per scene (direction tables, graph capture)
     for si in range(len(scenes)):
         w = (si, reqs[0][1][:256], sun0, [(vlb.LIGHT_DIRECTIONAL, (-sun0).tolist(), (3.0, 3.0, 3.0))])
@@ -593,7 +594,8 @@ def bench_c5(args):
                        "probes": P_all, "rays_per_probe": 4096,
                        "scenes": [{"scene": specs[i][0], "triangles": infos[i]["num_tris"],
                                    "bvh_depth": infos[i]["depth"]} for i in range(len(specs))],
-                       "state_budget_mib": args.state_budget, "step_rays": args.step_rays},
+                       "state_budget_mib": args.state_budget, "step_rays": args.step_rays,
+                       "policy": args.policy},
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
                            "p99": round(float(np.percentile(lat, 99)), 2),
                            "max": round(float(lat.max()), 2)},
@@ -636,7 +638,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "copy", "sweep"])
-    ap.add_argument("--rate", type=float, default=20.0, help="c5: request arrivals per second")
+    ap.add_argument("--rate", type=float, default=40.0, help="c5: request arrivals per second")
+    ap.add_argument("--policy", default="srpt", choices=sorted(SCHED), help="c5: server step scheduling")
     ap.add_argument("--requests", type=int, default=200, help="c5: number of requests")
     ap.add_argument("--state-budget", type=int, default=256, help="c5: server state arena (MiB)")
     ap.add_argument("--step-rays", type=int, default=1 << 25, help="c5: primary rays per step")

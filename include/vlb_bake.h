@@ -349,7 +349,7 @@ vlb_status  vlb_batch_destroy(vlb_batch* b);
  * free and their probe state (probes, partial sums, outputs) fits the
  * state_budget arena, and runs steps of at most step_rays primary rays that
  * hold a share of every admitted request's remaining work, two steps in
- * flight.  Each request has the semantics of vlb_bake and its coefficients
+ * flight (see VLB_SCHED_*).  Each request has the semantics of vlb_bake and its coefficients
  * do not depend on what else is served.  Scenes must stay resident while
  * requests on them are pending.  vlb_submit copies everything it needs. */
 typedef struct vlb_server vlb_server;
@@ -367,8 +367,14 @@ typedef struct vlb_server_options {
     uint32_t trace_variant;   /* 13..24; 0 = 20                           */
     uint32_t chunk;           /* rays per work fetch; 0 = 256             */
     uint32_t max_depth;       /* deepest scene BVH served; 0 = 64         */
-    uint32_t reserved[2];
+    uint32_t policy;          /* VLB_SCHED_*: how a step's rays are shared */
+    uint32_t reserved;
 } vlb_server_options;
+/* Step scheduling: SRPT fills each step with the admitted requests that have
+ * the least work left first (best median latency); SHARE gives every
+ * admitted request an equal share (processor sharing); FIFO serves them in
+ * arrival order. */
+enum { VLB_SCHED_SRPT = 0, VLB_SCHED_SHARE = 1, VLB_SCHED_FIFO = 2 };
 typedef struct vlb_request_info {
     double   submit_ms;       /* server clock: ms since vlb_server_create */
     double   admit_ms;        /* state allocated on the device            */

@@ -73,10 +73,11 @@ def test_server_request_equals_single_bake(ctx, scenes):
     srv.destroy()
 
 
-def test_server_alone_equals_in_mix(ctx, scenes):
+@pytest.mark.parametrize("policy", [vlb.SCHED_SRPT, vlb.SCHED_SHARE, vlb.SCHED_FIFO])
+def test_server_alone_equals_in_mix(ctx, scenes, policy):
     reqs = _requests(scenes, n=12, seed=5)
     target = 4
-    srv = vlb.Server(ctx, step_rays=1 << 15)
+    srv = vlb.Server(ctx, step_rays=1 << 15, policy=policy)
     alone, _ = srv.wait(srv.submit(*reqs[target]))
     ids = [srv.submit(sid, p) for sid, p in reqs]      # all in flight together
     res = [srv.wait(i) for i in ids]

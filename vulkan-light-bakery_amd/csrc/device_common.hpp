@@ -278,7 +278,9 @@ __device__ __forceinline__ BNode4 load_node(const BNode4* nodes, uint32_t i)
 // array indexed at run time would be placed in scratch memory).
 __device__ __forceinline__ int pick4(const int c[4], int k)
 {
-    return k == 0 ? c[0] : (k == 1 ? c[1] : (k == 2 ? c[2] : c[3]));
+    const int a = (k & 1) ? c[1] : c[0];
+    const int b = (k & 1) ? c[3] : c[2];
+    return (k & 2) ? b : a;
 }
 
 // Hit leaf children as a 4-bit mask; their entry distances are cleared so

@@ -10,10 +10,10 @@
 //    outputs) fits the device state arena (HBM budget) and their result fits
 //    the pinned result arena; nothing is allocated or freed with hipMalloc /
 //    hipFree while serving, so no step waits for the device to go idle;
-//  * continuous batching: every step holds the next projection tiles of every
-//    admitted request (an equal share each, the rest to the oldest), so a
-//    request that arrives while others are on the GPU starts at the next
-//    step and its work overlaps theirs;
+//  * continuous batching: every step is filled with the next projection tiles
+//    of the admitted requests -- least work left first (SRPT, the default),
+//    equal shares, or arrival order -- so a request that arrives while others
+//    are on the GPU starts at the next step and its work overlaps theirs;
 //  * two steps in flight: while the device runs step n, the host admits and
 //    stages step n+1 (its tables go through pinned staging buffers), and
 //    retires step n-1 (finished requests release their state).
@@ -412,14 +412,21 @@ void Server::run()
                 }
                 for (Upload& u : ups) u.src += upl0;
                 active_peak = std::max(active_peak, (uint32_t)active.size());
-                // processor sharing: an equal share of the step for every
-                // request with tiles left, the remainder to the oldest
+                // the step's trace tiles: SRPT (least work left first), FIFO
+                // (arrival order) or SHARE (equal shares, the rest to the
+                // oldest)
                 std::vector<Request*> runnable;
                 for (Request* r : active)
                     if (r->next_tile < r->js.ntiles) runnable.push_back(r);
+                auto left_tiles = [](const Request* r) {
+                    return (size_t)(r->js.ntiles - r->next_tile) * (size_t)r->js.nch;
+                };
+                if (opt.policy == VLB_SCHED_SRPT)
+                    std::stable_sort(runnable.begin(), runnable.end(),
+                                     [&](const Request* a, const Request* b) { return left_tiles(a) < left_tiles(b); });
                 std::vector<size_t> give(runnable.size(), 0);
                 size_t room = cap_tiles;
-                if (!runnable.empty()) {
+                if (opt.policy == VLB_SCHED_SHARE && !runnable.empty()) {
                     const size_t share = std::max<size_t>(1, cap_tiles / runnable.size());
                     for (size_t i = 0; i < runnable.size(); ++i) {
                         const Request* r = runnable[i];
@@ -429,14 +436,14 @@ void Server::run()
                         give[i] = n;
                         room -= n * nch;
                     }
-                    for (size_t i = 0; i < runnable.size() && room > 0; ++i) {
-                        const Request* r = runnable[i];
-                        const size_t nch = (size_t)r->js.nch;
-                        const size_t left = (size_t)(r->js.ntiles - r->next_tile) - give[i];
-                        const size_t n = std::min(left, room / nch);
-                        give[i] += n;
-                        room -= n * nch;
-                    }
+                }
+                for (size_t i = 0; i < runnable.size() && room > 0; ++i) {
+                    const Request* r = runnable[i];
+                    const size_t nch = (size_t)r->js.nch;
+                    const size_t left = (size_t)(r->js.ntiles - r->next_tile) - give[i];
+                    const size_t n = std::min(left, room / nch);
+                    give[i] += n;
+                    room -= n * nch;
                 }
                 const double t = now_ms();
                 for (size_t i = 0; i < runnable.size(); ++i) {
@@ -531,6 +538,7 @@ vlb_status vlb_server_create(vlb_ctx* h, const vlb_server_options* o, vlb_server
         VLB_REQUIRE(op.precision >= VLB_PREC_FP32 && op.precision <= VLB_PREC_FP16_FAST, VLB_ERR_INVALID,
                     "bad precision");
         VLB_REQUIRE(op.step_rays <= (1ull << 30), VLB_ERR_INVALID, "step_rays must be <= 2^30");
+        VLB_REQUIRE(op.policy <= VLB_SCHED_FIFO, VLB_ERR_INVALID, "bad scheduling policy");
         multi_variant(op.trace_variant, s->map, s->refill);
         s->chunk = (int)op.chunk;
         VLB_REQUIRE(s->chunk >= 64 && s->chunk <= 1024 && 1024 % s->chunk == 0, VLB_ERR_INVALID,

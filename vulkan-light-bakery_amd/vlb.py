@@ -38,6 +38,7 @@ LIGHT_DIRECTIONAL, LIGHT_POINT = 0, 1
 REF_SRGB, REF_UNORM8 = 1, 2
 EQUIRECT_SH_COMP, EQUIRECT_SKYBOX_COMP = 0, 1
 SCENE_ROOM, SCENE_TERRAIN, SCENE_CITY, SCENE_CUBE, SCENE_SPHERES = range(5)
+SCHED_SRPT, SCHED_SHARE, SCHED_FIFO = 0, 1, 2
 MAX_LIGHTS = 8
 
 
@@ -108,7 +109,7 @@ class ServerOptions(C.Structure):
                 ("result_bytes", C.c_uint64), ("upload_bytes", C.c_uint64),
                 ("max_active", C.c_uint32), ("max_lights", C.c_uint32), ("precision", C.c_int32),
                 ("trace_variant", C.c_uint32), ("chunk", C.c_uint32), ("max_depth", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("policy", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class RequestInfo(C.Structure):
@@ -568,12 +569,13 @@ class Server:
     def __init__(self, ctx: "Context", precision: int = PREC_FP32, state_budget: int = 0,
                  step_rays: int = 0, max_active: int = 0, max_lights: int = 0,
                  trace_variant: int = 0, chunk: int = 0, result_bytes: int = 0,
-                 upload_bytes: int = 0):
+                 upload_bytes: int = 0, policy: int = 0):
         self.ctx = ctx
         o = ServerOptions()
         o.state_budget, o.step_rays, o.max_active = state_budget, step_rays, max_active
         o.max_lights, o.precision, o.trace_variant, o.chunk = max_lights, precision, trace_variant, chunk
         o.result_bytes, o.upload_bytes = result_bytes, upload_bytes
+        o.policy = policy
         self.h = C.c_void_p()
         st = lib.vlb_server_create(ctx.h, C.byref(o), C.byref(self.h))
         if st:
