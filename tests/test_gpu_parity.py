@@ -80,6 +80,46 @@ def test_bvh_export_invariants(ctx, room):
     inner = ch[ch >= 0]
     assert len(np.unique(inner)) == len(inner) == nodes.shape[0] - 1
     assert (ch < 0).sum() == nt
+    # an inner child's box in its parent is the union of its own two child
+    # boxes; the root's two boxes together cover the whole scene
+    for a, slot in ((0, 0), (6, 1)):
+        c = ch[:, slot]
+        m = c >= 0
+        kid = nodes[c[m]]
+        ulo = np.minimum(kid[:, 0:3], kid[:, 6:9])
+        uhi = np.maximum(kid[:, 3:6], kid[:, 9:12])
+        assert np.array_equal(nodes[m, a:a + 3], ulo) and np.array_equal(nodes[m, a + 3:a + 6], uhi)
+    root_lo = np.minimum(nodes[0, 0:3], nodes[0, 6:9])
+    root_hi = np.maximum(nodes[0, 3:6], nodes[0, 9:12])
+    assert np.array_equal(root_lo, lo_t.min(axis=0)) and np.array_equal(root_hi, hi_t.max(axis=0))
+    assert ctx.scene_info(sid)["depth"] <= 64
+
+
+def test_bvh_coincident_and_degenerate_triangles(ctx):
+    """Thousands of copies of one triangle, and of a zero-area one, among a
+    few random ones: the build stays shallow (equal-area neighbours pair up
+    instead of chaining), and closest hits still go to the lowest prim id."""
+    rng = np.random.default_rng(9)
+    one = np.array([[0, 0, 0], [4, 0, 0], [0, 4, 0]], np.float32)
+    point = np.full((3, 3), 2.0, np.float32)
+    rand = rng.uniform(-6, 6, size=(300, 3, 3)).astype(np.float32)
+    tris = np.concatenate([np.repeat(one[None], 3000, 0), np.repeat(point[None], 3000, 0), rand])
+    tris = tris[rng.permutation(len(tris))]
+    xyz = tris.reshape(-1, 3)
+    idx = np.arange(len(xyz), dtype=np.uint32).reshape(-1, 3)
+    sid = ctx.upload_scene(xyz, idx)
+    assert ctx.scene_info(sid)["depth"] <= 48
+    o = orc.Scene(xyz, idx)
+    org = rng.uniform([-1, -1, -5], [5, 5, -1], size=(5000, 3)).astype(np.float32)
+    tgt = rng.uniform([-1, -1, -1], [5, 5, 1], size=(5000, 3)).astype(np.float32)
+    d = tgt - org
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    gp, gt, _, _ = ctx.trace_closest(sid, org, d, 0.0, 1e4)
+    cp, ct, _, _ = o.trace_closest(org, d, 0.0, 1e4)
+    assert (gp == cp).all(), f"prim mismatch on {(gp != cp).sum()} rays"
+    assert np.array_equal(gt.view(np.uint32), ct.view(np.uint32))
+    assert (cp >= 0).mean() > 0.3
+    ctx.release_scene(sid)
 
 
 def _bake_params(probes, rays=1024, **kw):

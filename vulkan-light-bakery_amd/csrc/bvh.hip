@@ -2,33 +2,18 @@
 // build of src/scene_manager.cpp:339-443; the reference hands triangles to
 // the driver with ePreferFastTrace and culling disabled).
 //
-// LBVH (Karras, "Maximizing Parallelism in the Construction of BVHs, Octrees,
-// and k-d Trees", HPG 2012): 30-bit Morton codes of triangle centroids,
-// radix sort, one thread per inner node finds its key range and split, then
-// every inner node's child boxes come from a segment-tree pyramid over the
-// sorted leaf boxes -- no atomics or inter-workgroup fences, so the build is
-// deterministic.  Node boxes are the exact min/max of their triangles'
-// vertices (conservative for the watertight test).
+// 30-bit Morton codes of triangle centroids, a stable radix sort, then a
+// PLOC hierarchy (agglomerative clustering over the Morton order, below)
+// with one triangle per leaf, collapsed to the 4-wide BVH the traversal
+// kernels walk.  Node boxes are unions of their children's boxes, i.e. the
+// exact min/max of their triangles' vertices (conservative for the
+// watertight test), and every step is deterministic.
 #include <hipcub/hipcub.hpp>
 
 #include "vlb_internal.hpp"
 
 namespace vlb {
 namespace {
-
-struct Box6 {
-    float lo[3], hi[3];
-};
-
-__device__ __forceinline__ void box_union(Box6& a, const float4& lo, const float4& hi)
-{
-    a.lo[0] = fminf(a.lo[0], lo.x);
-    a.lo[1] = fminf(a.lo[1], lo.y);
-    a.lo[2] = fminf(a.lo[2], lo.z);
-    a.hi[0] = fmaxf(a.hi[0], hi.x);
-    a.hi[1] = fmaxf(a.hi[1], hi.y);
-    a.hi[2] = fmaxf(a.hi[2], hi.z);
-}
 
 // Gather triangles, per-prim boxes and centroids, shading records.
 __global__ void k_gather(const float* __restrict__ xyz, const uint32_t* __restrict__ idx,
@@ -181,65 +166,27 @@ __global__ void k_morton(const float4* __restrict__ cen, int n, const float* __r
     vals[i] = (uint32_t)i;
 }
 
-__device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int n, int i, int j)
-{
-    if (j < 0 || j >= n) return -1;
-    uint32_t a = keys[i], b = keys[j];
-    if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
-    return __clz(a ^ b);
-}
+// ------------------------------------------------------------- PLOC
+// Meister and Bittner, "Parallel Locally-Ordered Clustering for Bounding
+// Volume Hierarchy Construction", IEEE TVCG 24(3), 2018.  Clusters start as
+// the Morton-ordered leaves; each pass every cluster finds the neighbour
+// within kPlocRadius positions whose union box has the least surface area,
+// mutual pairs merge into a new inner node at the lower position, and the
+// cluster list is compacted.  Candidate pairs are ordered by (area, distance
+// in positions, parity of the lower position, lower position): a strict total
+// order, so the least pair of a pass is always mutual and every pass merges
+// at least once; and where areas tie (coincident or degenerate triangles),
+// neighbours pair up as (0,1), (2,3), ... instead of chaining, so such runs
+// halve each pass.  The build is deterministic (no atomics; positions and
+// node ids come from one scan).  Node ids are handed out downward, so the
+// final merge -- the root -- is node 0.
+constexpr int kPlocRadius = 16;
 
-// One thread per inner node i in [0, n-2] (Karras 2012, Figure 4).
-// Writes children into nodes[i].d, the covered leaf range, and the parent
-// link (2*parent + slot) of both children.
-__global__ void k_karras(const uint32_t* __restrict__ keys, int n, BNode* __restrict__ nodes,
-                         int2* __restrict__ range, int* __restrict__ iparent,
-                         int* __restrict__ lparent)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-    int dmin = delta(keys, n, i, i - d);
-    int lmax = 2;
-    while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
-    int l = 0;
-    for (int t = lmax / 2; t >= 1; t /= 2)
-        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-    int j = i + l * d;
-    int dnode = delta(keys, n, i, j);
-    int s = 0;
-    int t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-    } while (t > 1);
-    int gamma = i + s * d + (d < 0 ? d : 0);
-    int first = min(i, j), last = max(i, j);
-    int c0, c1;
-    if (first == gamma) {
-        c0 = ~gamma;
-        lparent[gamma] = 2 * i + 0;
-    } else {
-        c0 = gamma;
-        iparent[gamma] = 2 * i + 0;
-    }
-    if (last == gamma + 1) {
-        c1 = ~(gamma + 1);
-        lparent[gamma + 1] = 2 * i + 1;
-    } else {
-        c1 = gamma + 1;
-        iparent[gamma + 1] = 2 * i + 1;
-    }
-    nodes[i].d = make_int4(c0, c1, 0, 0);
-    range[i] = make_int2(first, last);
-    if (i == 0) iparent[0] = -1;
-}
-
-// Leaf-order triangles and level 0 of the box pyramid.
+// Leaf-order triangles and the initial clusters (leaf j = ~j).
 __global__ void k_leaves(const uint32_t* __restrict__ order, int n,
                          const float4* __restrict__ tri_raw, const float4* __restrict__ lo,
                          const float4* __restrict__ hi, float4* __restrict__ tris,
-                         float4* __restrict__ plo, float4* __restrict__ phi)
+                         float4* __restrict__ clo, float4* __restrict__ chi, int* __restrict__ cid)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -247,90 +194,157 @@ __global__ void k_leaves(const uint32_t* __restrict__ order, int n,
     tris[3 * j + 0] = tri_raw[3 * p + 0];
     tris[3 * j + 1] = tri_raw[3 * p + 1];
     tris[3 * j + 2] = tri_raw[3 * p + 2];
-    plo[j] = lo[p];
-    phi[j] = hi[p];
+    clo[j] = lo[p];
+    chi[j] = hi[p];
+    cid[j] = ~j;
 }
 
-// Pyramid level k from level k-1 (m_prev entries).
-__global__ void k_pyramid(const float4* __restrict__ lo_prev, const float4* __restrict__ hi_prev,
-                          int m_prev, float4* __restrict__ lo, float4* __restrict__ hi)
+// Candidate j = i + o against the best so far.  A NaN area (non-finite
+// input) counts as +inf so the pair order stays total.  Tie key for this i:
+// distance, then the lower position's parity, then the lower position (the
+// left candidate is the lower one).
+__device__ __forceinline__ void ploc_consider(const float4& a, const float4& b, const float4& l,
+                                              const float4& h, int i, int o, float& best,
+                                              int& bj, int& bk)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int m = (m_prev + 1) >> 1;
+    const float dx = fmaxf(b.x, h.x) - fminf(a.x, l.x);
+    const float dy = fmaxf(b.y, h.y) - fminf(a.y, l.y);
+    const float dz = fmaxf(b.z, h.z) - fminf(a.z, l.z);
+    float ar = dx * dy + dy * dz + dz * dx;
+    ar = ar == ar ? ar : __builtin_huge_valf();
+    const int k = 4 * abs(o) + 2 * (min(i, i + o) & 1) + (o > 0 ? 1 : 0);
+    if (bj < 0 || ar < best || (ar == best && k < bk)) {
+        best = ar;
+        bj = i + o;
+        bk = k;
+    }
+}
+
+// Nearest neighbour of every cluster within kPlocRadius positions; the
+// block's window of boxes is staged in LDS.
+__global__ __launch_bounds__(256) void k_ploc_nn(const float4* __restrict__ clo,
+                                                 const float4* __restrict__ chi, int m,
+                                                 int* __restrict__ nn)
+{
+    constexpr int W = 256 + 2 * kPlocRadius;
+    __shared__ float4 slo[W], shi[W];
+    const int base = blockIdx.x * 256;
+    for (int t = threadIdx.x; t < W; t += 256) {
+        const int g = base - kPlocRadius + t;
+        if (g >= 0 && g < m) {
+            slo[t] = clo[g];
+            shi[t] = chi[g];
+        }
+    }
+    __syncthreads();
+    const int i = base + threadIdx.x;
     if (i >= m) return;
-    float4 l = lo_prev[2 * i], h = hi_prev[2 * i];
-    if (2 * i + 1 < m_prev) {
-        float4 l2 = lo_prev[2 * i + 1], h2 = hi_prev[2 * i + 1];
-        l = make_float4(fminf(l.x, l2.x), fminf(l.y, l2.y), fminf(l.z, l2.z), 0.0f);
-        h = make_float4(fmaxf(h.x, h2.x), fmaxf(h.y, h2.y), fmaxf(h.z, h2.z), 0.0f);
-    }
-    lo[i] = l;
-    hi[i] = h;
+    const int li = threadIdx.x + kPlocRadius;
+    const float4 a = slo[li], b = shi[li];
+    float best = __builtin_huge_valf();
+    int bj = -1, bk = 0;
+    const int o0 = max(-kPlocRadius, -i), o1 = min(kPlocRadius, m - 1 - i);
+    for (int o = o0; o <= o1; ++o)
+        if (o != 0) ploc_consider(a, b, slo[li + o], shi[li + o], i, o, best, bj, bk);
+    nn[i] = bj;
 }
 
-struct Pyramid {
-    const float4* lo[32];
-    const float4* hi[32];
-    int levels;
-};
-
-__device__ Box6 range_box(const Pyramid& py, int a, int b)
+// The last passes (m <= kPlocFull clusters, the top of the tree): exact
+// nearest neighbour over all clusters, one workgroup.
+constexpr int kPlocFull = 1024;
+__global__ __launch_bounds__(kPlocFull) void k_ploc_nn_full(const float4* __restrict__ clo,
+                                                            const float4* __restrict__ chi,
+                                                            int m, int* __restrict__ nn)
 {
-    Box6 r;
-    for (int k = 0; k < 3; ++k) {
-        r.lo[k] = __builtin_huge_valf();
-        r.hi[k] = -__builtin_huge_valf();
+    __shared__ float4 slo[kPlocFull], shi[kPlocFull];
+    const int i = threadIdx.x;
+    if (i < m) {
+        slo[i] = clo[i];
+        shi[i] = chi[i];
     }
-    // Segment-tree walk: level k entry i covers leaves [i*2^k, (i+1)*2^k).
-    // An odd left end or an even right end is taken alone; what remains is
-    // [even, odd], whose parents are [a/2, b/2] one level up.
-    int lvl = 0;
-    while (a <= b && lvl < py.levels) {
-        if (a & 1) {
-            box_union(r, py.lo[lvl][a], py.hi[lvl][a]);
-            ++a;
-        }
-        if (a <= b && !(b & 1)) {
-            box_union(r, py.lo[lvl][b], py.hi[lvl][b]);
-            --b;
-        }
-        if (a > b) break;
-        a >>= 1;
-        b >>= 1;
-        ++lvl;
-    }
-    return r;
+    __syncthreads();
+    if (i >= m) return;
+    const float4 a = slo[i], b = shi[i];
+    float best = __builtin_huge_valf();
+    int bj = -1, bk = 0;
+    for (int j = 0; j < m; ++j)
+        if (j != i) ploc_consider(a, b, slo[j], shi[j], i, j - i, best, bj, bk);
+    nn[i] = bj;
 }
 
-__global__ void k_node_boxes(BNode* __restrict__ nodes, int nin, const int2* __restrict__ range,
-                             Pyramid py)
+// Per cluster: low word 1 if it survives the pass (unpaired, or the lower
+// of a mutual pair), high word 1 if it is the lower of a mutual pair.
+__global__ void k_ploc_flags(const int* __restrict__ nn, int m,
+                             unsigned long long* __restrict__ flag)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nin) return;
-    int4 c = nodes[i].d;
-    Box6 b[2];
-    const int ch[2] = {c.x, c.y};
-    for (int k = 0; k < 2; ++k) {
-        if (ch[k] < 0) {
-            int leaf = ~ch[k];
-            float4 l = py.lo[0][leaf], h = py.hi[0][leaf];
-            b[k].lo[0] = l.x;
-            b[k].lo[1] = l.y;
-            b[k].lo[2] = l.z;
-            b[k].hi[0] = h.x;
-            b[k].hi[1] = h.y;
-            b[k].hi[2] = h.z;
-        } else {
-            int2 r = range[ch[k]];
-            b[k] = range_box(py, r.x, r.y);
-        }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int j = nn[i];
+    const bool mutual = nn[j] == i;
+    unsigned long long f = 0;
+    if (!mutual) f = 1ull;
+    else if (i < j) f = (1ull << 32) | 1ull;
+    flag[i] = f;
+}
+
+// Compaction and merge.  scan = inclusive sum of the flags; with M merges
+// in a pass over m clusters, the pass's nodes are [m - 1 - M, m - 1).
+__global__ void k_ploc_merge(const int* __restrict__ nn, int m,
+                             const unsigned long long* __restrict__ scan,
+                             const float4* __restrict__ clo, const float4* __restrict__ chi,
+                             const int* __restrict__ cid, float4* __restrict__ olo,
+                             float4* __restrict__ ohi, int* __restrict__ ocid,
+                             BNode* __restrict__ nodes, int* __restrict__ iparent,
+                             int* __restrict__ lparent)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int j = nn[i];
+    const bool mutual = nn[j] == i;
+    if (mutual && j < i) return;  // absorbed by its partner
+    const unsigned long long f = mutual ? (1ull << 32) | 1ull : 1ull;
+    const unsigned long long ex = scan[i] - f;
+    const int pos = (int)(uint32_t)ex;
+    if (!mutual) {
+        olo[pos] = clo[i];
+        ohi[pos] = chi[i];
+        ocid[pos] = cid[i];
+        return;
     }
+    const int merges = (int)(scan[m - 1] >> 32);
+    const int id = m - 1 - merges + (int)(ex >> 32);
+    const float4 l0 = clo[i], h0 = chi[i], l1 = clo[j], h1 = chi[j];
+    const int c0 = cid[i], c1 = cid[j];
     BNode n;
-    n.a = make_float4(b[0].lo[0], b[0].lo[1], b[0].lo[2], b[0].hi[0]);
-    n.b = make_float4(b[0].hi[1], b[0].hi[2], b[1].lo[0], b[1].lo[1]);
-    n.c = make_float4(b[1].lo[2], b[1].hi[0], b[1].hi[1], b[1].hi[2]);
-    n.d = c;
-    nodes[i] = n;
+    n.a = make_float4(l0.x, l0.y, l0.z, h0.x);
+    n.b = make_float4(h0.y, h0.z, l1.x, l1.y);
+    n.c = make_float4(l1.z, h1.x, h1.y, h1.z);
+    n.d = make_int4(c0, c1, 0, 0);
+    nodes[id] = n;
+    if (c0 >= 0) iparent[c0] = 2 * id + 0;
+    else lparent[~c0] = 2 * id + 0;
+    if (c1 >= 0) iparent[c1] = 2 * id + 1;
+    else lparent[~c1] = 2 * id + 1;
+    olo[pos] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.0f);
+    ohi[pos] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.0f);
+    ocid[pos] = id;
+}
+
+// One triangle: a root whose two children are the same leaf; testing it
+// twice cannot change a result.
+__global__ void k_single(const float4* __restrict__ clo, const float4* __restrict__ chi,
+                         BNode* __restrict__ nodes, int* __restrict__ iparent,
+                         int* __restrict__ lparent)
+{
+    const float4 l = clo[0], h = chi[0];
+    BNode n;
+    n.a = make_float4(l.x, l.y, l.z, h.x);
+    n.b = make_float4(h.y, h.z, l.x, l.y);
+    n.c = make_float4(l.z, h.x, h.y, h.z);
+    n.d = make_int4(~0, ~0, 0, 0);
+    nodes[0] = n;
+    iparent[0] = -1;
+    lparent[0] = 0;
 }
 
 // Depth = inner nodes on the longest root-to-leaf path.
@@ -483,55 +497,68 @@ void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
     VLB_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, keys.as<uint32_t>(),
                                                keys2.as<uint32_t>(), vals.as<uint32_t>(),
                                                vals2.as<uint32_t>(), n, 0, 30, st));
-    // hierarchy
+    // hierarchy (PLOC), one host round trip per pass for the cluster count
     const int nin = n > 1 ? n - 1 : 1;
     s.nodes = dev_alloc(ctx, (size_t)nin * sizeof(BNode));
     s.tris = dev_alloc(ctx, (size_t)n * 3 * sizeof(float4));
-    DevBuf range = dev_alloc(ctx, (size_t)nin * sizeof(int2));
     DevBuf iparent = dev_alloc(ctx, (size_t)nin * sizeof(int));
     DevBuf lparent = dev_alloc(ctx, (size_t)n * sizeof(int));
-    // pyramid storage
-    std::vector<DevBuf> plo, phi;
-    std::vector<int> sizes;
-    for (int m = n;; m = (m + 1) >> 1) {
-        sizes.push_back(m);
-        plo.push_back(dev_alloc(ctx, (size_t)m * sizeof(float4)));
-        phi.push_back(dev_alloc(ctx, (size_t)m * sizeof(float4)));
-        if (m == 1) break;
+    DevBuf clo[2], chi[2], cid[2];
+    for (int k = 0; k < 2; ++k) {
+        clo[k] = dev_alloc(ctx, (size_t)n * sizeof(float4));
+        chi[k] = dev_alloc(ctx, (size_t)n * sizeof(float4));
+        cid[k] = dev_alloc(ctx, (size_t)n * sizeof(int));
     }
-    VLB_REQUIRE(sizes.size() <= 32, VLB_ERR_INVALID, "scene too large for the box pyramid");
     k_leaves<<<nblk(n), 256, 0, st>>>(vals2.as<uint32_t>(), n, tri_raw.as<float4>(),
                                       lo.as<float4>(), hi.as<float4>(), s.tris.as<float4>(),
-                                      plo[0].as<float4>(), phi[0].as<float4>());
-    for (size_t k = 1; k < sizes.size(); ++k)
-        k_pyramid<<<nblk(sizes[k]), 256, 0, st>>>(plo[k - 1].as<float4>(), phi[k - 1].as<float4>(),
-                                                  sizes[k - 1], plo[k].as<float4>(),
-                                                  phi[k].as<float4>());
+                                      clo[0].as<float4>(), chi[0].as<float4>(), cid[0].as<int>());
     VLB_HIP(hipGetLastError());
-    Pyramid py{};
-    py.levels = (int)sizes.size();
-    for (size_t k = 0; k < sizes.size(); ++k) {
-        py.lo[k] = plo[k].as<float4>();
-        py.hi[k] = phi[k].as<float4>();
-    }
     DevBuf dep = dev_alloc(ctx, sizeof(unsigned int));
     VLB_HIP(hipMemsetAsync(dep.p, 0, sizeof(unsigned int), st));
     if (n == 1) {
-        // single triangle: one inner node whose two children are the same
-        // leaf; testing it twice cannot change a result.
-        BNode h;
-        h.d = make_int4(~0, ~0, 0, 0);
-        VLB_HIP(hipMemcpyAsync(s.nodes.p, &h, sizeof h, hipMemcpyHostToDevice, st));
-        int2 r0 = make_int2(0, 0);
-        VLB_HIP(hipMemcpyAsync(range.p, &r0, sizeof r0, hipMemcpyHostToDevice, st));
-        k_node_boxes<<<1, 256, 0, st>>>(s.nodes.as<BNode>(), 1, range.as<int2>(), py);
+        k_single<<<1, 1, 0, st>>>(clo[0].as<float4>(), chi[0].as<float4>(), s.nodes.as<BNode>(),
+                                  iparent.as<int>(), lparent.as<int>());
         unsigned int one = 1;
         VLB_HIP(hipMemcpyAsync(dep.p, &one, sizeof one, hipMemcpyHostToDevice, st));
     } else {
-        k_karras<<<nblk(nin), 256, 0, st>>>(keys2.as<uint32_t>(), n, s.nodes.as<BNode>(),
-                                            range.as<int2>(), iparent.as<int>(),
-                                            lparent.as<int>());
-        k_node_boxes<<<nblk(nin), 256, 0, st>>>(s.nodes.as<BNode>(), nin, range.as<int2>(), py);
+        DevBuf nn = dev_alloc(ctx, (size_t)n * sizeof(int));
+        DevBuf flag = dev_alloc(ctx, (size_t)n * sizeof(unsigned long long));
+        DevBuf scan = dev_alloc(ctx, (size_t)n * sizeof(unsigned long long));
+        size_t ps_bytes = 0;
+        VLB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, ps_bytes, flag.as<unsigned long long>(),
+                                                 scan.as<unsigned long long>(), n, st));
+        DevBuf ps_tmp = dev_alloc(ctx, ps_bytes > 0 ? ps_bytes : 16);
+        int m = n, cur = 0;
+        // every pass merges at least one pair, so at most n - 1 passes
+        for (int pass = 0; m > 1; ++pass) {
+            VLB_REQUIRE(pass < n, VLB_ERR_DEVICE, "PLOC build made no progress");
+            if (m <= kPlocFull)
+                k_ploc_nn_full<<<1, kPlocFull, 0, st>>>(clo[cur].as<float4>(), chi[cur].as<float4>(),
+                                                        m, nn.as<int>());
+            else
+                k_ploc_nn<<<nblk(m), 256, 0, st>>>(clo[cur].as<float4>(), chi[cur].as<float4>(), m,
+                                                   nn.as<int>());
+            k_ploc_flags<<<nblk(m), 256, 0, st>>>(nn.as<int>(), m, flag.as<unsigned long long>());
+            VLB_HIP(hipcub::DeviceScan::InclusiveSum(ps_tmp.p, ps_bytes,
+                                                     flag.as<unsigned long long>(),
+                                                     scan.as<unsigned long long>(), m, st));
+            k_ploc_merge<<<nblk(m), 256, 0, st>>>(
+                nn.as<int>(), m, scan.as<unsigned long long>(), clo[cur].as<float4>(),
+                chi[cur].as<float4>(), cid[cur].as<int>(), clo[cur ^ 1].as<float4>(),
+                chi[cur ^ 1].as<float4>(), cid[cur ^ 1].as<int>(), s.nodes.as<BNode>(),
+                iparent.as<int>(), lparent.as<int>());
+            VLB_HIP(hipGetLastError());
+            unsigned long long tot = 0;
+            VLB_HIP(hipMemcpyAsync(&tot, scan.as<unsigned long long>() + (m - 1), sizeof tot,
+                                   hipMemcpyDeviceToHost, st));
+            VLB_HIP(hipStreamSynchronize(st));
+            const int m2 = (int)(uint32_t)tot;
+            VLB_REQUIRE(m2 >= 1 && m2 < m, VLB_ERR_DEVICE, "PLOC build made no progress");
+            m = m2;
+            cur ^= 1;
+        }
+        const int root_parent = -1;
+        VLB_HIP(hipMemcpyAsync(iparent.p, &root_parent, sizeof root_parent, hipMemcpyHostToDevice, st));
         k_depth<<<nblk(n), 256, 0, st>>>(iparent.as<int>(), lparent.as<int>(), n,
                                          dep.as<unsigned int>());
     }
