@@ -477,6 +477,30 @@ vlb_status  vlb_probe_lookup(vlb_ctx* ctx, uint32_t scene, const vlb_probe_grid_
                              const float* points, const float* normals, uint32_t n,
                              float* out_rgb);
 
+/* Multi-bounce indirect light: the step after direct-only baking that the
+ * reference names as its goal (README.md:9-11), in the form its renderer
+ * shades surfaces with a baked grid (shaders/main.rchit:118-172: albedo x
+ * (lookup + direct)).  After this call every execute of a LAMBERT plan adds,
+ * at each primary hit, albedo x scale x the grid lookup of vlb_probe_lookup
+ * at the hit point and the two-sided shading normal (IRRADIANCE mode with
+ * scale 1 is the Lambertian bounce: albedo x E / pi), reading a source grid
+ * kept on the device; the execute then copies its own result into that
+ * source, so N executes are N further bounces.  The grid describes the
+ * plan's own probes: count product = num_probes, num_coefs = K, order and
+ * origin/step as they were laid out.  Visibility rays to the corner probes
+ * are traced like shadow rays (and counted as such).
+ * coeffs: [P][K][3] host floats to start from, or NULL to start from the
+ * plan's last result (the plan must have executed).  ip NULL: back to
+ * direct light only.  No counterpart call in the reference. */
+typedef struct vlb_indirect_params {
+    vlb_probe_grid_desc grid;
+    vlb_lookup_params   lookup;
+    float               scale;     /* factor on the looked-up value          */
+    uint32_t            reserved[3];
+} vlb_indirect_params;
+vlb_status  vlb_plan_set_indirect(vlb_plan* plan, const vlb_indirect_params* ip,
+                                  const float* coeffs);
+
 /* ------------------------------------------------------- device buffers */
 vlb_status  vlb_device_alloc(vlb_ctx* ctx, size_t bytes, void** dptr);
 vlb_status  vlb_device_free(vlb_ctx* ctx, void* dptr);

@@ -97,6 +97,11 @@ typedef struct {
     volatile uint32_t next;
     uint64_t shadow_total;
     pthread_mutex_t mtx;
+    /* multi-bounce: source grid of the indirect term, or ig == NULL */
+    const vlb_probe_grid_desc* ig;
+    const float* icoeffs;
+    const vlb_lookup_params* ilp;
+    float iscale;
 } bake_job;
 
 /* Radiance of one ray.  rad_f: float pipeline; rad_d: double values with
@@ -247,6 +252,17 @@ static void shade_ray(bake_job* J, const float o[3], const float d[3], float rad
         rad_f[c] = alb[c] * acc[c] * INV_PI_F;
         rad_d[c] = (double)alb[c] * accd[c] / ORC_PI_D;
     }
+    if (J->ig) {
+        /* indirect term: albedo x scale x the grid lookup at the hit point
+         * and the two-sided float normal (orc_probe_lookup's arithmetic) */
+        double E[3];
+        *nshadow += (uint64_t)orc_lookup_point(s, J->ig, J->icoeffs, J->ilp, P, N, E);
+        for (int c = 0; c < 3; ++c) {
+            const double e = (double)J->iscale * E[c];
+            rad_f[c] = rad_f[c] + alb[c] * (float)e;
+            rad_d[c] += (double)alb[c] * e;
+        }
+    }
 }
 
 static void bake_probe(bake_job* J, uint32_t pi, uint64_t* nshadow)
@@ -298,8 +314,10 @@ static void* worker(void* arg)
     return NULL;
 }
 
-int orc_bake(const orc_scene* s, const vlb_bake_params* p, int mode64, int nthreads, int brute,
-             double* out, float* radiance, int32_t* hit, uint64_t* shadow_rays)
+static int bake_common(const orc_scene* s, const vlb_bake_params* p, const vlb_probe_grid_desc* ig,
+                       const float* icoeffs, const vlb_lookup_params* ilp, float iscale, int mode64,
+                       int nthreads, int brute, double* out, float* radiance, int32_t* hit,
+                       uint64_t* shadow_rays)
 {
     if (!s || !p || !out || !p->probe_xyz) return -1;
     if (p->sh_order != 2 && p->sh_order != 3) return -1;
@@ -356,6 +374,10 @@ int orc_bake(const orc_scene* s, const vlb_bake_params* p, int mode64, int nthre
     J.out = out;
     J.radiance = radiance;
     J.hit = hit;
+    J.ig = ig;
+    J.icoeffs = icoeffs;
+    J.ilp = ilp;
+    J.iscale = iscale;
     pthread_mutex_init(&J.mtx, NULL);
     if (nthreads <= 0) nthreads = (int)sysconf(_SC_NPROCESSORS_ONLN);
     if (nthreads < 1) nthreads = 1;
@@ -373,4 +395,24 @@ int orc_bake(const orc_scene* s, const vlb_bake_params* p, int mode64, int nthre
     free(dirs);
     free(wts);
     return 0;
+}
+
+int orc_bake(const orc_scene* s, const vlb_bake_params* p, int mode64, int nthreads, int brute,
+             double* out, float* radiance, int32_t* hit, uint64_t* shadow_rays)
+{
+    return bake_common(s, p, NULL, NULL, NULL, 0.0f, mode64, nthreads, brute, out, radiance, hit,
+                       shadow_rays);
+}
+
+int orc_bake_indirect(const orc_scene* s, const vlb_bake_params* p, const vlb_probe_grid_desc* g,
+                      const float* coeffs, const vlb_lookup_params* lp, float scale, int mode64,
+                      int nthreads, int brute, double* out, float* radiance, int32_t* hit,
+                      uint64_t* shadow_rays)
+{
+    if (!g || !coeffs || !lp || !p || orc_check_grid(g)) return -1;
+    if (p->shade_mode != VLB_SHADE_LAMBERT) return -1;
+    if ((uint64_t)g->count[0] * g->count[1] * g->count[2] != p->num_probes) return -1;
+    if (g->num_coefs != (p->sh_order == 2 ? 9u : 16u)) return -1;
+    return bake_common(s, p, g, coeffs, lp, scale, mode64, nthreads, brute, out, radiance, hit,
+                       shadow_rays);
 }

@@ -96,6 +96,22 @@ void orc_trace_occluded_batch(const orc_scene* s, const float* org, const float*
 int orc_bake(const orc_scene* s, const vlb_bake_params* p, int mode64, int nthreads,
              int brute, double* out, float* radiance, int32_t* hit,
              uint64_t* shadow_rays);
+/* orc_bake with the multi-bounce indirect term of vlb_plan_set_indirect: at
+ * each LAMBERT hit, radiance += albedo x scale x orc_probe_lookup(g, coeffs,
+ * lp) at the hit point and the two-sided float shading normal; the lookup's
+ * visibility rays count as shadow rays.  g must describe the bake's own
+ * probes (count product = num_probes, num_coefs = K).
+ * Pinned by: (1) zero albedo -> orc_bake exactly; (2) a constant source grid
+ * (c00 only) in a closed room: result - orc_bake = albedo x scale x c00 x
+ * Y00 x the projection of a constant over the same rays; (3) an l = 1 source
+ * field L(d) = d.y over a floor whose triangles face away from the probes:
+ * result = albedo x (2/3) x (N.y of the normal facing the ray) projected over
+ * the downward rays (closed form of the clamped-cosine convolution and the
+ * two-sided normal). */
+int orc_bake_indirect(const orc_scene* s, const vlb_bake_params* p, const vlb_probe_grid_desc* g,
+                      const float* coeffs, const vlb_lookup_params* lp, float scale, int mode64,
+                      int nthreads, int brute, double* out, float* radiance, int32_t* hit,
+                      uint64_t* shadow_rays);
 
 /* Procedural sky of the benchmark configurations (replaces main.rmiss). */
 void orc_sky(const vlb_sky* sky, const double d[3], double rgb[3]);

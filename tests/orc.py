@@ -36,6 +36,9 @@ def _load():
         "orc_trace_occluded_batch": (None, [VP, F, F, F, U32, C.c_float, I32, C.POINTER(C.c_uint8)]),
         "orc_bake": (I32, [VP, C.POINTER(vlb.BakeParams), I32, I32, I32, D, F, C.POINTER(I32),
                            C.POINTER(C.c_uint64)]),
+        "orc_bake_indirect": (I32, [VP, C.POINTER(vlb.BakeParams), C.POINTER(vlb.ProbeGridDesc), F,
+                                    C.POINTER(vlb.LookupParams), C.c_float, I32, I32, I32, D, F,
+                                    C.POINTER(I32), C.POINTER(C.c_uint64)]),
         "orc_sky": (None, [C.POINTER(vlb.Sky), D, D]),
         "orc_probe_lookup": (I32, [VP, C.POINTER(vlb.ProbeGridDesc), F, C.POINTER(vlb.LookupParams),
                                    F, F, U32, D]),
@@ -148,6 +151,24 @@ class Scene:
                         C.byref(nsh))
         if st:
             raise ValueError("orc_bake rejected the parameters")
+        return {"coeffs": out, "radiance": rad, "hits": hit, "shadow_rays": nsh.value}
+
+    def bake_indirect(self, params: "vlb.Params", grid, coeffs, lp, scale=1.0, mode64=True,
+                      nthreads=0, want_radiance=False, want_hits=False):
+        """orc_bake plus the multi-bounce indirect term (vlb_plan_set_indirect)."""
+        p = params.to_c()
+        P, K, R = p.num_probes, params.K, params.R
+        c = f32(coeffs)
+        out = np.zeros((P, K, 3), np.float64)
+        rad = np.zeros((P, R, 3), np.float32) if want_radiance else None
+        hit = np.zeros((P, R), np.int32) if want_hits else None
+        nsh = C.c_uint64(0)
+        st = L.orc_bake_indirect(self.h, C.byref(p), C.byref(grid), _f(c), C.byref(lp), scale,
+                                 int(mode64), nthreads, 0, _d(out), None if rad is None else _f(rad),
+                                 None if hit is None else hit.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 C.byref(nsh))
+        if st:
+            raise ValueError("orc_bake_indirect rejected the parameters")
         return {"coeffs": out, "radiance": rad, "hits": hit, "shadow_rays": nsh.value}
 
     def __del__(self):

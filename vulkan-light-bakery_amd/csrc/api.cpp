@@ -253,6 +253,24 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
 
 void plan_sync(Plan& pl);
 
+// Multi-bounce: the result just merged becomes the next execute's source.
+void refresh_indirect(Plan& pl, hipStream_t st)
+{
+    if (pl.indirect)
+        launch_copy_f32(pl.ind.as<float>(), pl.out.as<float>(), (size_t)pl.args.P * pl.K * 3, st);
+}
+
+// Drops a captured graph (kernel arguments changed).
+void drop_graph(Plan& pl)
+{
+    if (pl.graph_exec) {
+        (void)hipGraphExecDestroy(pl.graph_exec);
+        (void)hipGraphDestroy(pl.graph);
+        pl.graph_exec = nullptr;
+        pl.graph = nullptr;
+    }
+}
+
 void plan_launch(Plan& pl)
 {
     hipStream_t st = pl.ctx->stream;
@@ -265,6 +283,7 @@ void plan_launch(Plan& pl)
                 launch_trace_stage(pl, false, false, st);
                 launch_project(pl.args, pl.precision, pl.pgrid, st);
                 launch_merge(pl.margs, st);
+                refresh_indirect(pl, st);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(st, &g);
@@ -284,6 +303,7 @@ void plan_launch(Plan& pl)
         launch_project(pl.args, pl.precision, pl.pgrid, st);
         if (ev) VLB_HIP(hipEventRecord(ev[2], st));
         launch_merge(pl.margs, st);
+        refresh_indirect(pl, st);
         if (ev) VLB_HIP(hipEventRecord(ev[3], st));
     }
     if (ev) ++pl.timed_pending;
@@ -310,6 +330,50 @@ std::vector<float> solid_angle_table(int N)
 }
 
 }  // namespace
+
+namespace vlb {
+// Validates a probe grid and lookup parameters; returns the probe count.
+uint64_t check_grid(const vlb_probe_grid_desc& g, const vlb_lookup_params& lp)
+{
+    VLB_REQUIRE(g.num_coefs == 9 || g.num_coefs == 16, VLB_ERR_INVALID, "num_coefs must be 9 or 16");
+    VLB_REQUIRE(g.order == VLB_ORDER_LINEAR || g.order == VLB_ORDER_REFERENCE_BFS, VLB_ERR_INVALID,
+                "bad probe order");
+    VLB_REQUIRE(lp.mode == VLB_LOOKUP_REFERENCE || lp.mode == VLB_LOOKUP_IRRADIANCE, VLB_ERR_INVALID,
+                "bad lookup mode");
+    VLB_REQUIRE(std::isfinite(lp.bias), VLB_ERR_INVALID, "lookup bias must be finite");
+    uint64_t P = 1;
+    for (int a = 0; a < 3; ++a) {
+        VLB_REQUIRE(g.count[a] >= 1, VLB_ERR_INVALID, "grid counts must be >= 1");
+        VLB_REQUIRE(g.step[a] != 0.0f && std::isfinite(g.step[a]) && std::isfinite(g.origin[a]),
+                    VLB_ERR_INVALID, "grid origin and step must be finite, step non-zero");
+        P *= g.count[a];
+    }
+    VLB_REQUIRE(P < (1ull << 28), VLB_ERR_INVALID, "grid too large");
+    return P;
+}
+
+ProbeGridView grid_view(const vlb_probe_grid_desc& g, const vlb_lookup_params& lp,
+                        const float* dcoeffs, float scale)
+{
+    ProbeGridView v;
+    memset(&v, 0, sizeof v);
+    v.coeffs = dcoeffs;
+    for (int ax = 0; ax < 3; ++ax) {
+        v.origin[ax] = g.origin[ax];
+        v.step[ax] = g.step[ax];
+        v.count[ax] = g.count[ax];
+    }
+    v.order = g.order;
+    v.K = (int)g.num_coefs;
+    v.exact = g.sh_constants == VLB_SH_EXACT;
+    v.mode = lp.mode;
+    v.vis = lp.visibility ? 1 : 0;
+    v.bias = lp.bias;
+    v.wmax = sqrtf(g.step[0] * g.step[0] + g.step[1] * g.step[1] + g.step[2] * g.step[2]);
+    v.scale = scale;
+    return v;
+}
+}  // namespace vlb
 
 extern "C" {
 
@@ -614,6 +678,49 @@ vlb_status vlb_plan_use_graph(vlb_plan* h, int32_t on)
     return VLB_OK;
 }
 
+vlb_status vlb_plan_set_indirect(vlb_plan* h, const vlb_indirect_params* ip, const float* coeffs)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Plan* pl = reinterpret_cast<Plan*>(h);
+    std::lock_guard<std::recursive_mutex> lk(pl->ctx->mtx);
+    return guard(pl->ctx, [&] {
+        Context& ctx = *pl->ctx;
+        BakeArgs& a = pl->args;
+        const int nvis0 = (int)pl->params.num_lights;
+        if (!ip) {
+            memset(&a.sp.ind, 0, sizeof a.sp.ind);
+            if (pl->params.shade_mode == VLB_SHADE_LAMBERT) a.nvis = nvis0;
+            pl->indirect = false;
+            drop_graph(*pl);
+            return;
+        }
+        VLB_REQUIRE(pl->params.shade_mode == VLB_SHADE_LAMBERT, VLB_ERR_UNSUPPORTED,
+                    "indirect light needs LAMBERT shading");
+        const uint64_t P = check_grid(ip->grid, ip->lookup);
+        VLB_REQUIRE(P == (uint64_t)a.P, VLB_ERR_INVALID, "grid probe count differs from the plan's");
+        VLB_REQUIRE((int)ip->grid.num_coefs == pl->K, VLB_ERR_INVALID, "grid num_coefs differs from the plan's K");
+        VLB_REQUIRE(std::isfinite(ip->scale), VLB_ERR_INVALID, "scale must be finite");
+        VLB_REQUIRE(coeffs || pl->ran, VLB_ERR_INVALID,
+                    "no source grid: pass coefficients or execute the plan first");
+        VLB_HIP(hipSetDevice(ctx.device));
+        const size_t n = (size_t)P * pl->K * 3;
+        if (!pl->ind.p) pl->ind = dev_alloc(ctx, n * sizeof(float));
+        if (coeffs) VLB_HIP(hipMemcpyAsync(pl->ind.p, coeffs, n * sizeof(float), hipMemcpyHostToDevice, ctx.stream));
+        else launch_copy_f32(pl->ind.as<float>(), pl->out.as<float>(), n, ctx.stream);
+        const int nvis = nvis0 + (ip->lookup.visibility ? 8 : 0);
+        if (pl->hits.p && pl->vis.bytes < (size_t)a.ntrace * 512 * (size_t)nvis) {
+            pl->vis = dev_alloc(ctx, (size_t)a.ntrace * 512 * (size_t)nvis);
+            a.vis = pl->vis.as<uint8_t>();
+        }
+        a.nvis = nvis;
+        a.sp.ind = grid_view(ip->grid, ip->lookup, pl->ind.as<float>(), ip->scale);
+        pl->indirect = true;
+        drop_graph(*pl);
+        // a host source buffer must stay valid until the copy is done
+        VLB_HIP(hipStreamSynchronize(ctx.stream));
+    });
+}
+
 vlb_status vlb_plan_count(vlb_plan* h)
 {
     if (!h) return VLB_ERR_INVALID;
@@ -883,19 +990,7 @@ vlb_status vlb_probe_lookup(vlb_ctx* h, uint32_t scene, const vlb_probe_grid_des
     Context* c = reinterpret_cast<Context*>(h);
     std::lock_guard<std::recursive_mutex> lk(c->mtx);
     return guard(c, [&] {
-        VLB_REQUIRE(g->num_coefs == 9 || g->num_coefs == 16, VLB_ERR_INVALID, "num_coefs must be 9 or 16");
-        VLB_REQUIRE(g->order == VLB_ORDER_LINEAR || g->order == VLB_ORDER_REFERENCE_BFS, VLB_ERR_INVALID,
-                    "bad probe order");
-        VLB_REQUIRE(lp->mode == VLB_LOOKUP_REFERENCE || lp->mode == VLB_LOOKUP_IRRADIANCE, VLB_ERR_INVALID,
-                    "bad lookup mode");
-        uint64_t P = 1;
-        for (int a = 0; a < 3; ++a) {
-            VLB_REQUIRE(g->count[a] >= 1, VLB_ERR_INVALID, "grid counts must be >= 1");
-            VLB_REQUIRE(g->step[a] != 0.0f && std::isfinite(g->step[a]), VLB_ERR_INVALID,
-                        "grid step must be finite and non-zero");
-            P *= g->count[a];
-        }
-        VLB_REQUIRE(P < (1ull << 28), VLB_ERR_INVALID, "grid too large");
+        const uint64_t P = check_grid(*g, *lp);
         VLB_REQUIRE(n < (1u << 28), VLB_ERR_INVALID, "too many lookup points");
         if (n == 0) return;
         Scene& s = find_scene(*c, scene);
@@ -914,19 +1009,7 @@ vlb_status vlb_probe_lookup(vlb_ctx* h, uint32_t scene, const vlb_probe_grid_des
         LookupArgs a;
         memset(&a, 0, sizeof a);
         a.sc = s.view();
-        for (int ax = 0; ax < 3; ++ax) {
-            a.origin[ax] = g->origin[ax];
-            a.step[ax] = g->step[ax];
-            a.count[ax] = g->count[ax];
-        }
-        a.order = g->order;
-        a.K = (int)g->num_coefs;
-        a.exact = g->sh_constants == VLB_SH_EXACT;
-        a.mode = lp->mode;
-        a.vis = lp->visibility ? 1 : 0;
-        a.bias = lp->bias;
-        a.wmax = sqrtf(g->step[0] * g->step[0] + g->step[1] * g->step[1] + g->step[2] * g->step[2]);
-        a.coeffs = d_c.as<float>();
+        a.g = grid_view(*g, *lp, d_c.as<float>(), 1.0f);
         a.pts = d_p.as<float>();
         a.nrm = d_n.as<float>();
         a.n = (int)n;

@@ -37,7 +37,8 @@ namespace {
 // (tile_map, a.map) sets how many neighbouring probes x directions (probes
 // Morton sorted, directions on a Hilbert curve) one wave covers.
 // OCC > 0 asks the compiler for OCC waves per SIMD (VGPR budget 512/OCC).
-template <bool RECORD, bool COUNT, int OCC>
+// IND: plans with the multi-bounce term.
+template <bool RECORD, bool COUNT, int OCC, bool IND>
 __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
 {
     __shared__ uint32_t lds[kLdsDepth * 256];
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
         if (probe < a.P && dd.w != 0.0f) {
             const float4 pp = a.probes[probe];
             int hp = -1;
-            L = shade_ray<COUNT>(a.sc, a.sp, make_float3(pp.x, pp.y, pp.z),
+            L = shade_ray<COUNT, IND>(a.sc, a.sp, make_float3(pp.x, pp.y, pp.z),
                                  make_float3(dd.x, dd.y, dd.z), a.tmin, a.tmax, st, cnt, nshadow, hp);
             if (RECORD) {
                 const size_t ri = (size_t)a.probe_perm[probe] * a.R + a.dir_perm[r];
@@ -188,20 +189,24 @@ int occupancy_of(K kernel, int cus)
 int trace_grid_size(Context& ctx, int occ)
 {
     int g;
-    if (occ == 8) g = occupancy_of(k_trace<false, false, 8>, ctx.num_cus);
-    else if (occ == 6) g = occupancy_of(k_trace<false, false, 6>, ctx.num_cus);
-    else g = occupancy_of(k_trace<false, false, 0>, ctx.num_cus);
+    if (occ == 8) g = occupancy_of(k_trace<false, false, 8, false>, ctx.num_cus);
+    else if (occ == 6) g = occupancy_of(k_trace<false, false, 6, false>, ctx.num_cus);
+    else g = occupancy_of(k_trace<false, false, 0, false>, ctx.num_cus);
     return (g + 7) & ~7;
 }
 
 void launch_trace(const BakeArgs& a, bool record, bool count, int occ, int grid, hipStream_t st)
 {
     VLB_REQUIRE(grid % 8 == 0, VLB_ERR_INVALID, "trace grid must be a multiple of 8");
-    if (record) k_trace<true, false, 0><<<grid, 256, 0, st>>>(a);
-    else if (count) k_trace<false, true, 0><<<grid, 256, 0, st>>>(a);
-    else if (occ == 8) k_trace<false, false, 8><<<grid, 256, 0, st>>>(a);
-    else if (occ == 6) k_trace<false, false, 6><<<grid, 256, 0, st>>>(a);
-    else k_trace<false, false, 0><<<grid, 256, 0, st>>>(a);
+    if (a.sp.ind.coeffs) {
+        if (record) k_trace<true, false, 0, true><<<grid, 256, 0, st>>>(a);
+        else if (count) k_trace<false, true, 0, true><<<grid, 256, 0, st>>>(a);
+        else k_trace<false, false, 0, true><<<grid, 256, 0, st>>>(a);
+    } else if (record) k_trace<true, false, 0, false><<<grid, 256, 0, st>>>(a);
+    else if (count) k_trace<false, true, 0, false><<<grid, 256, 0, st>>>(a);
+    else if (occ == 8) k_trace<false, false, 8, false><<<grid, 256, 0, st>>>(a);
+    else if (occ == 6) k_trace<false, false, 6, false><<<grid, 256, 0, st>>>(a);
+    else k_trace<false, false, 0, false><<<grid, 256, 0, st>>>(a);
     VLB_HIP(hipGetLastError());
 }
 
@@ -239,6 +244,22 @@ void launch_trace_occluded(const SceneView& sc, const float* org, const float* d
                            uint32_t stride, int grid, hipStream_t st)
 {
     k_trace_occluded<<<grid, 256, 0, st>>>(sc, org, dir, tmax, n, tmin, occ, spill, stride);
+    VLB_HIP(hipGetLastError());
+}
+
+namespace {
+__global__ void k_copy_f32(float* __restrict__ dst, const float* __restrict__ src, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+}  // namespace
+
+void launch_copy_f32(float* dst, const float* src, size_t n, hipStream_t st)
+{
+    if (n == 0) return;
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 4096);
+    k_copy_f32<<<grid, 256, 0, st>>>(dst, src, n);
     VLB_HIP(hipGetLastError());
 }
 

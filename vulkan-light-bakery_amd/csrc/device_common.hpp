@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include "../../include/vlb_bake.h"
+#include "probe_grid.hpp"
 
 namespace vlb {
 
@@ -68,6 +69,9 @@ struct ShadeParams {
     vlb_light    lights[VLB_MAX_LIGHTS];
     vlb_sky      sky;
     vlb_ref_shading ref;
+    // LAMBERT multi-bounce: a baked grid whose looked-up value at the hit
+    // (times scale) adds albedo x that value; ind.coeffs null: direct only
+    ProbeGridView ind;
 };
 
 constexpr int kLdsDepth = 16;  // stack entries kept in LDS per lane
@@ -454,8 +458,45 @@ __device__ __forceinline__ float unorm8(float x)
 }
 
 // ---------------------------------------------------------------- shading
-// Radiance carried by one primary ray.  `nshadow` counts traced shadow rays.
+// Indirect term of a multi-bounce bake at hit point P with the two-sided
+// shading normal N: the grid lookup of probe_grid.hpp (visibility rays to the
+// corner probes traced here, counted as shadow rays), times g.scale.
 template <bool COUNT>
+__device__ float3 grid_indirect(const SceneView& sc, const ProbeGridView& g, const float3& P,
+                                const float3& N, const Stack& st, TraceCount& cnt,
+                                uint32_t& nshadow)
+{
+    const float p[3] = {P.x, P.y, P.z};
+    const float n[3] = {N.x, N.y, N.z};
+    float Y[16];
+    grid_basis(g, n, Y);
+    float sw = 0.0f, acc[3] = {0.0f, 0.0f, 0.0f};
+    for (int v = 0; v < 8; ++v) {
+        const GridCorner c = grid_corner(g, p, v);
+        float w = c.w;
+        if (corner_needs_ray(g, c, sc.ntris)) {
+            ++nshadow;
+            const float3 org = make_float3(P.x + g.bias * N.x, P.y + g.bias * N.y, P.z + g.bias * N.z);
+            const float3 dir = make_float3(c.dir[0] / c.dist, c.dir[1] / c.dist, c.dir[2] / c.dist);
+            if (trace_any<COUNT>(sc, org, dir, 0.0f, c.dist, st, cnt)) w = 0.0f;
+        }
+        if (w > 0.0f) {
+            float val[3];
+            corner_value(g, c.row, Y, val);
+            acc[0] += w * val[0];
+            acc[1] += w * val[1];
+            acc[2] += w * val[2];
+            sw += w;
+        }
+    }
+    if (!(sw > 0.0f)) return make_float3(0.f, 0.f, 0.f);
+    return make_float3(g.scale * (acc[0] / sw), g.scale * (acc[1] / sw), g.scale * (acc[2] / sw));
+}
+
+// Radiance carried by one primary ray.  `nshadow` counts traced shadow rays.
+// IND compiles the multi-bounce term in (used only when sp.ind.coeffs is
+// set); direct-only kernels are built without it and keep its registers free.
+template <bool COUNT, bool IND = false>
 __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const float3& o,
                             const float3& d, float tmin, float tmax, const Stack& st,
                             TraceCount& cnt, uint32_t& nshadow, int& hit_prim)
@@ -569,7 +610,11 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
         }
     }
     const float kInvPi = 0.318309886183790671538f;
-    return make_float3(alb.x * acc.x * kInvPi, alb.y * acc.y * kInvPi, alb.z * acc.z * kInvPi);
+    if (!IND || !sp.ind.coeffs)
+        return make_float3(alb.x * acc.x * kInvPi, alb.y * acc.y * kInvPi, alb.z * acc.z * kInvPi);
+    const float3 E = grid_indirect<COUNT>(sc, sp.ind, P, N, st, cnt, nshadow);
+    return make_float3(alb.x * acc.x * kInvPi + alb.x * E.x, alb.y * acc.y * kInvPi + alb.y * E.y,
+                       alb.z * acc.z * kInvPi + alb.z * E.z);
 }
 
 }  // namespace vlb

@@ -2,7 +2,7 @@
 // normalisations and the literal 6-figure constants of shaders/sh_common.h.
 #pragma once
 
-#include "vlb_internal.hpp"
+#include <hip/hip_runtime.h>
 
 namespace vlb {
 

@@ -12,6 +12,7 @@
 // an LDS combine across the 4 waves, one partial per workgroup, then an FP64
 // merge kernel that adds the partials in a fixed order.  No float atomics,
 // so results are deterministic (sh.comp's shared "+=" is a data race).
+#include "vlb_internal.hpp"
 #include "sh_eval.hpp"
 
 namespace vlb {

@@ -265,6 +265,10 @@ struct Plan {
     double sum_trace = 0.0, sum_project = 0.0, sum_merge = 0.0, sum_total = 0.0;
     bool ran = false;
     vlb_bake_stats stats{};
+    // multi-bounce: source grid of the indirect term (a copy of the last
+    // result, refreshed at the end of every execute)
+    DevBuf ind;
+    bool indirect = false;
     ~Plan();
 };
 
@@ -330,13 +334,14 @@ struct MultiArgs {
 };
 
 // ------------------------------------------------------ probe-grid lookup
+// Validates a probe grid and lookup parameters; returns the probe count.
+uint64_t check_grid(const vlb_probe_grid_desc& g, const vlb_lookup_params& lp);
+ProbeGridView grid_view(const vlb_probe_grid_desc& g, const vlb_lookup_params& lp,
+                        const float* dcoeffs, float scale);
+
 struct LookupArgs {
     SceneView sc;
-    float origin[3], step[3];
-    uint32_t count[3];
-    int order, K, exact, mode, vis;
-    float bias, wmax;
-    const float* coeffs;      // [P][K][3]
+    ProbeGridView g;
     const float* pts;         // n x 3
     const float* nrm;         // n x 3
     int n;
@@ -354,6 +359,7 @@ void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
 int  trace_grid_size(Context& ctx, int occ);
 void launch_trace(const BakeArgs& a, bool record, bool count, int occ, int grid, hipStream_t stream);
 void launch_project(const BakeArgs& a, int precision, int grid, hipStream_t stream);
+void launch_copy_f32(float* dst, const float* src, size_t n, hipStream_t stream);
 // wave.hip
 int  wave_grid_size(Context& ctx);
 void launch_wave(const BakeArgs& a, bool record, bool count, int grid, hipStream_t stream);

@@ -74,7 +74,7 @@ constexpr int kChunk = 256;
 
 // Sets up item `it` for this lane.  Returns false when the item needs no
 // traversal (its result is written here).
-template <bool ANY>
+template <bool ANY, bool CORNERS>
 __device__ __forceinline__ bool item_setup(const BakeArgs& a, size_t it, TravState& T, int& slot_out,
                                            int& light_out)
 {
@@ -111,7 +111,7 @@ __device__ __forceinline__ bool item_setup(const BakeArgs& a, size_t it, TravSta
     const float3 o = make_float3(pp.x, pp.y, pp.z), d = make_float3(dd.x, dd.y, dd.z);
     const HitGeom g = hit_geom(a.sc.shade, o, d, h.x, __int_as_float(h.y), __int_as_float(h.z),
                                __int_as_float(h.w));
-    const ShadowRay s = shadow_ray(a.sp, g, d, j);
+    const ShadowRay s = shadow_ray<CORNERS>(a.sp, g, d, j);
     if (!s.needed) {
         *vis = 0;
         return false;
@@ -120,7 +120,9 @@ __device__ __forceinline__ bool item_setup(const BakeArgs& a, size_t it, TravSta
     return true;
 }
 
-template <bool ANY, bool COUNT>
+// CORNERS (shadow pass only): the item space also holds the corner
+// visibility rays of the multi-bounce term.
+template <bool ANY, bool COUNT, bool CORNERS>
 __global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigned long long* ctr)
 {
     __shared__ uint32_t lds[kLdsDepth * 256];
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigne
                 const bool mine = !act && rank < avail;
                 const size_t take = min((size_t)__popcll(idle), avail);
                 if (mine) {
-                    act = item_setup<ANY>(a, pnext + rank, T, slot, light);
+                    act = item_setup<ANY, CORNERS>(a, pnext + rank, T, slot, light);
                     if (act && ANY) ++ntraced;
                 }
                 pnext += take;
@@ -201,7 +203,7 @@ __global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigne
 
 // Per ray: sky on a miss, else the shading of env_map.rchit (REFERENCE) or
 // Lambert, with the recorded visibility; radiance in MFMA operand order.
-template <bool RECORD>
+template <bool RECORD, bool IND>
 __global__ __launch_bounds__(256) void k_shade(BakeArgs a)
 {
     const size_t nslots = (size_t)a.ntrace * 512;
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(256) void k_shade(BakeArgs a)
         if (h.x != -2) {
             const float4 dd = a.dirs[r];
             const float3 d = make_float3(dd.x, dd.y, dd.z);
-            L = shade_record(a.sp, a.sc.shade, a.probes + probe, d, h, a.vis + slot * a.nvis);
+            L = shade_record<IND>(a.sp, a.sc.shade, a.probes + probe, d, h, a.vis + slot * a.nvis);
             if (RECORD) {
                 const size_t ri = (size_t)a.probe_perm[probe] * a.R + a.dir_perm[r];
                 if (a.rad_out) {
@@ -251,8 +253,8 @@ int occupancy_of(K kernel, int cus)
 
 int wave_grid_size(Context& ctx)
 {
-    const int g = std::max(occupancy_of(k_rays<false, false>, ctx.num_cus),
-                           occupancy_of(k_rays<true, false>, ctx.num_cus));
+    const int g = std::max(occupancy_of(k_rays<false, false, false>, ctx.num_cus),
+                           occupancy_of(k_rays<true, false, false>, ctx.num_cus));
     return (g + 7) & ~7;
 }
 
@@ -263,18 +265,31 @@ void launch_wave(const BakeArgs& a, bool record, bool count, int grid, hipStream
     // work counters: [0..7] primary, [8..15] shadow (cleared by a kernel so
     // the sequence can be captured in a graph)
     k_zero_u64<<<1, 64, 0, st>>>(a.wctr, 16);
-    if (count) k_rays<false, true><<<grid, 256, 0, st>>>(a, nslots, a.wctr);
-    else k_rays<false, false><<<grid, 256, 0, st>>>(a, nslots, a.wctr);
+    if (count) k_rays<false, true, false><<<grid, 256, 0, st>>>(a, nslots, a.wctr);
+    else k_rays<false, false, false><<<grid, 256, 0, st>>>(a, nslots, a.wctr);
     VLB_HIP(hipGetLastError());
     const size_t nshadow = nslots * (size_t)a.nvis;
     if (a.nvis > 0) {
-        if (count) k_rays<true, true><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8);
-        else k_rays<true, false><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8);
+        const bool corners = a.sp.ind.coeffs && a.sp.ind.vis;
+        if (corners) {
+            if (count) k_rays<true, true, true><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8);
+            else k_rays<true, false, true><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8);
+        } else if (count) {
+            k_rays<true, true, false><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8);
+        } else {
+            k_rays<true, false, false><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8);
+        }
         VLB_HIP(hipGetLastError());
     }
     const int sgrid = (int)std::min<size_t>((nslots + 255) / 256, 8192);
-    if (record) k_shade<true><<<sgrid, 256, 0, st>>>(a);
-    else k_shade<false><<<sgrid, 256, 0, st>>>(a);
+    if (a.sp.ind.coeffs) {
+        if (record) k_shade<true, true><<<sgrid, 256, 0, st>>>(a);
+        else k_shade<false, true><<<sgrid, 256, 0, st>>>(a);
+    } else if (record) {
+        k_shade<true, false><<<sgrid, 256, 0, st>>>(a);
+    } else {
+        k_shade<false, false><<<sgrid, 256, 0, st>>>(a);
+    }
     VLB_HIP(hipGetLastError());
 }
 

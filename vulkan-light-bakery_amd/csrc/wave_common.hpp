@@ -44,6 +44,9 @@ struct ShadowRay {
     bool needed;
 };
 
+// CORNERS: j >= nlights are the visibility rays of the multi-bounce term
+// (compiled only into the kernels that trace them).
+template <bool CORNERS = false>
 __device__ __forceinline__ ShadowRay shadow_ray(const ShadeParams& sp, const HitGeom& g,
                                                 const float3& d, int j)
 {
@@ -67,6 +70,21 @@ __device__ __forceinline__ ShadowRay shadow_ray(const ShadeParams& sp, const Hit
     }
     float3 N = g.N;
     if (dot3(N, d) > 0.0f) N = make_float3(-N.x, -N.y, -N.z);
+    if (CORNERS && j >= sp.nlights) {
+        // visibility ray to corner probe j - nlights of the indirect grid
+        // (only when the grid asks for them; the hit implies a scene)
+        const float p[3] = {g.P.x, g.P.y, g.P.z};
+        const GridCorner c = grid_corner(sp.ind, p, j - sp.nlights);
+        s.needed = corner_needs_ray(sp.ind, c, 1);
+        s.org = make_float3(g.P.x + sp.ind.bias * N.x, g.P.y + sp.ind.bias * N.y,
+                            g.P.z + sp.ind.bias * N.z);
+        s.dist = c.dist;
+        s.dir = s.needed ? make_float3(c.dir[0] / c.dist, c.dir[1] / c.dist, c.dir[2] / c.dist)
+                         : make_float3(0.f, 0.f, 1.f);
+        s.E = make_float3(0.f, 0.f, 0.f);
+        s.cosv = 0.0f;
+        return s;
+    }
     const vlb_light& Lj = sp.lights[j];
     if (Lj.type == VLB_LIGHT_POINT) {
         const float3 Lv = make_float3(Lj.vec[0] - g.P.x, Lj.vec[1] - g.P.y, Lj.vec[2] - g.P.z);
@@ -168,9 +186,38 @@ __device__ __forceinline__ float3 sky_for(const ShadeParams& sp, const float3& d
     return L;
 }
 
+// grid_indirect of device_common.hpp with the corner visibility rays already
+// traced: vis[v] != 0 marks corner v occluded (read only where a ray was
+// needed).
+__device__ __forceinline__ float3 grid_indirect_rec(const ProbeGridView& g, const float3& P,
+                                                    const float3& N, const uint8_t* vis)
+{
+    const float p[3] = {P.x, P.y, P.z};
+    const float n[3] = {N.x, N.y, N.z};
+    float Y[16];
+    grid_basis(g, n, Y);
+    float sw = 0.0f, acc[3] = {0.0f, 0.0f, 0.0f};
+    for (int v = 0; v < 8; ++v) {
+        const GridCorner c = grid_corner(g, p, v);
+        float w = c.w;
+        if (corner_needs_ray(g, c, 1) && vis[v] != 0) w = 0.0f;
+        if (w > 0.0f) {
+            float val[3];
+            corner_value(g, c.row, Y, val);
+            acc[0] += w * val[0];
+            acc[1] += w * val[1];
+            acc[2] += w * val[2];
+            sw += w;
+        }
+    }
+    if (!(sw > 0.0f)) return make_float3(0.f, 0.f, 0.f);
+    return make_float3(g.scale * (acc[0] / sw), g.scale * (acc[1] / sw), g.scale * (acc[2] / sw));
+}
+
 // Radiance of a primary ray from direction d, origin *probe, with hit
 // record h (prim >= 0, t, u, v as bits; -1 miss) and the visibility bytes
 // vis[j] of its shadow rays (env_map.rchit:73-101 / Lambert).
+template <bool IND = false>
 __device__ __forceinline__ float3 shade_record(const ShadeParams& sp, const float4* shade,
                                                const float4* probe, const float3& d,
                                                const int4& h, const uint8_t* vis)
@@ -220,7 +267,15 @@ __device__ __forceinline__ float3 shade_record(const ShadeParams& sp, const floa
         }
     }
     const float kInvPi = 0.318309886183790671538f;
-    return make_float3(g.alb.x * acc.x * kInvPi, g.alb.y * acc.y * kInvPi, g.alb.z * acc.z * kInvPi);
+    if (!IND || !sp.ind.coeffs)
+        return make_float3(g.alb.x * acc.x * kInvPi, g.alb.y * acc.y * kInvPi,
+                           g.alb.z * acc.z * kInvPi);
+    float3 N = g.N;
+    if (dot3(N, d) > 0.0f) N = make_float3(-N.x, -N.y, -N.z);
+    const float3 E = grid_indirect_rec(sp.ind, g.P, N, vis + sp.nlights);
+    return make_float3(g.alb.x * acc.x * kInvPi + g.alb.x * E.x,
+                       g.alb.y * acc.y * kInvPi + g.alb.y * E.y,
+                       g.alb.z * acc.z * kInvPi + g.alb.z * E.z);
 }
 
 }  // namespace vlb

@@ -153,6 +153,11 @@ def lookup_params(mode=0, visibility=True, bias=1e-3) -> LookupParams:
     return lp
 
 
+class IndirectParams(C.Structure):
+    _fields_ = [("grid", ProbeGridDesc), ("lookup", LookupParams), ("scale", C.c_float),
+                ("reserved", C.c_uint32 * 3)]
+
+
 class HostScene(C.Structure):
     _fields_ = [("nverts", C.c_uint32), ("ntris", C.c_uint32), ("xyz", C.POINTER(C.c_float)),
                 ("normals", C.POINTER(C.c_float)), ("idx", C.POINTER(C.c_uint32)),
@@ -214,6 +219,7 @@ def _load(path: str = LIB_PATH) -> C.CDLL:
         "vlb_project_equirect": (I32, [VP, P(C.c_uint8), U32, U32, I32, F]),
         "vlb_sh_reconstruct": (I32, [VP, F, U32, U32, P(C.c_uint8)]),
         "vlb_probe_lookup": (I32, [VP, U32, P(ProbeGridDesc), F, P(LookupParams), F, F, U32, F]),
+        "vlb_plan_set_indirect": (I32, [VP, P(IndirectParams), F]),
         "vlb_device_alloc": (I32, [VP, SZ, P(VP)]),
         "vlb_device_free": (I32, [VP, VP]),
         "vlb_upload": (I32, [VP, VP, VP, SZ]),
@@ -496,6 +502,27 @@ class Plan:
         lib.vlb_plan_stats(self.h, C.byref(s))
         return {k: getattr(s, k) for k, _ in BakeStats._fields_}
 
+    def set_indirect(self, grid: "ProbeGridDesc" = None, lp: "LookupParams" = None, scale=1.0,
+                     coeffs=None):
+        """Multi-bounce (vlb_plan_set_indirect): from now on every execute adds
+        albedo x scale x the grid lookup at each primary hit, reading the
+        previous execute's result (or `coeffs` for the first one).  grid None:
+        back to direct light only."""
+        if grid is None:
+            self._chk(lib.vlb_plan_set_indirect(self.h, None, None), "plan_set_indirect")
+            return
+        ip = IndirectParams()
+        ip.grid = grid
+        ip.lookup = lp if lp is not None else lookup_params(LOOKUP_IRRADIANCE, False)
+        ip.scale = scale
+        c = None
+        if coeffs is not None:
+            c = np.ascontiguousarray(coeffs, dtype=np.float32)
+            if c.size != self.P * self.K * 3:
+                raise ValueError("coeffs must hold P x K x 3 floats")
+        self._chk(lib.vlb_plan_set_indirect(self.h, C.byref(ip), None if c is None else _fptr(c)),
+                  "plan_set_indirect")
+
     def destroy(self):
         if self.h:
             lib.vlb_plan_destroy(self.h)
@@ -730,6 +757,23 @@ class Context:
         h = C.c_void_p()
         self._chk(lib.vlb_plan_create(self.h, sid, C.byref(p), C.byref(h)), "plan_create")
         return Plan(self, h, p.num_probes, params.K)
+
+    def bake_bounces(self, sid: int, params: Params, grid: "ProbeGridDesc", bounces: int,
+                     lp: "LookupParams" = None, scale=1.0) -> list:
+        """Direct bake, then `bounces` indirect passes over the plan's own grid;
+        returns the coefficients after each pass (direct first)."""
+        plan = self.plan(sid, params)
+        try:
+            plan.execute()
+            out = [plan.download()]
+            if bounces > 0:
+                plan.set_indirect(grid, lp, scale)
+                for _ in range(bounces):
+                    plan.execute()
+                    out.append(plan.download())
+            return out
+        finally:
+            plan.destroy()
 
     def batch(self, jobs, max_round_rays: int = 0, trace_variant: int = 0, chunk: int = 0,
               keep_f64: bool = False) -> Batch:
