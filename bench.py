@@ -14,7 +14,10 @@ roofline figure is for the trace stage, timed with hipEvents on the
 library's own stream inside the timed region.
 
     python bench.py [--steps K] [--warmup W] [--precision fp32|bf16|fp16]
-                    [--config c2|c1|c3|c4|c5|copy|sweep] [--no-cpu-baseline]
+                    [--config c2|c1|c3|c4|c5|copy|sweep|bounce] [--no-cpu-baseline]
+
+--no-cpu-baseline skips the CPU leg (the oracle's baseline timing and the
+SH error check against the FP64 oracle).
 """
 import argparse
 import json
@@ -49,6 +52,20 @@ def load_traffic(name):
             return json.load(f)
     except (OSError, ValueError):
         return None
+
+
+def oracle_leg(args):
+    """The CPU leg of a run: the CPU oracle (tests/orc.py over oracle/),
+    used for the CPU baseline timing and, as the checker, for the SH error
+    against FP64 -- never for what is measured on the GPU.  None with
+    --no-cpu-baseline, which skips both.  This is the only place bench.py
+    loads the oracle."""
+    if args.no_cpu_baseline:
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+
+    return orc
 
 
 def bench_c2(args):
@@ -100,20 +117,19 @@ def bench_c2(args):
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None}
 
-    # error of this run against the FP64 oracle on a seeded probe subset
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import orc
-
-    rng = np.random.default_rng(0)
-    sub = np.sort(rng.choice(P, size=64, replace=False))
-    psub = vlb.config_c2_params(scene, seed=2)
-    psub.probes = params.probes[sub].copy()
-    oscene = orc.Scene.from_dict(scene)
-    ref = oscene.bake(psub, mode64=True, nthreads=min(16, os.cpu_count() or 1))["coeffs"]
-    err = rel_err(coeffs[sub], ref)
-
-    cpu = None
-    if not args.no_cpu_baseline:
+    # CPU leg: error of this run against the FP64 oracle on a seeded probe
+    # subset, and the CPU baseline
+    orc = oracle_leg(args)
+    err_line, cpu = None, None
+    if orc is not None:
+        rng = np.random.default_rng(0)
+        sub = np.sort(rng.choice(P, size=64, replace=False))
+        psub = vlb.config_c2_params(scene, seed=2)
+        psub.probes = params.probes[sub].copy()
+        oscene = orc.Scene.from_dict(scene)
+        ref = oscene.bake(psub, mode64=True, nthreads=min(16, os.cpu_count() or 1))["coeffs"]
+        err = rel_err(coeffs[sub], ref)
+        err_line = {"max": float(err.max()), "p99": float(np.percentile(err, 99)), "probes": len(sub)}
         # the FP32 C oracle on the same scene, probes and rays: whole c2
         # bakes repeated until >= 10 s of wall time on the host's cores
         nthreads = min(16, os.cpu_count() or 1)
@@ -157,8 +173,7 @@ def bench_c2(args):
         "trace_variant": variant,
         "kernel_ms": {"trace": round(st["trace_ms"], 4), "project": round(st["project_ms"], 4),
                       "merge": round(st["merge_ms"], 4), "timed_runs": st["timed_runs"]},
-        "sh_rel_err_vs_fp64": {"max": float(err.max()), "p99": float(np.percentile(err, 99)),
-                               "probes": len(sub)},
+        "sh_rel_err_vs_fp64": err_line,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -174,12 +189,9 @@ def bench_c1(args):
 
     import vlb
 
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import orc
-
     N, B = 128, 1024
     rng = np.random.default_rng(1)
-    base = np.stack([orc.procedural_cubemap(N, vlb.sun_direction(rng)) for _ in range(8)])
+    base = np.stack([vlb.procedural_cubemap(N, vlb.sun_direction(rng)) for _ in range(8)])
     ctx = vlb.Context(0)
     nbytes = B * 6 * N * N * 16
     d_in = ctx.alloc(nbytes)
@@ -201,8 +213,8 @@ def bench_c1(args):
         proj.append(ctx.skybox_last_ms()[0])
     out = np.zeros((B, 9, 3), np.float32)
     ctx.download(d_out, out)
-    ref = orc.skybox_sh(base[0], 9)
-    err = rel_err(out[0], ref)
+    orc = oracle_leg(args)
+    err = rel_err(out[0], orc.skybox_sh(base[0], 9)).max() if orc is not None else None
     kms = float(np.mean(proj))
     achieved = (nbytes + B * 27 * 4) / (kms / 1e3) / 1e9
     line = {"metric": "skyboxes/s (6x128x128 RGBA32F -> L2)", "value": round(B * args.steps / (t1 - t0), 1),
@@ -210,7 +222,7 @@ def bench_c1(args):
             "ms_per_step": round(1e3 * (t1 - t0) / args.steps, 4), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "c1 skybox SH", "batch": B, "face": N},
-            "sh_rel_err_vs_fp64": float(err.max()),
+            "sh_rel_err_vs_fp64": None if err is None else float(err),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
             "cpu_baseline": None}
@@ -257,9 +269,6 @@ def bench_c3(args):
 
     import vlb
 
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import orc
-
     scene = vlb.generate_scene(vlb.SCENE_TERRAIN, 3)
     ctx = vlb.Context(0)
     sid = ctx.upload_scene_dict(scene)
@@ -299,9 +308,10 @@ def bench_c3(args):
         plan.destroy()
     # per-band error vs FP64 on a seeded probe subset (the oracle is the checker)
     sub = np.sort(rng.choice(len(probes), size=24, replace=False))
-    osc = orc.Scene.from_dict(scene)
+    orc = oracle_leg(args)
+    osc = orc.Scene.from_dict(scene) if orc is not None else None
     errs = {}
-    for order in (2, 3):
+    for order in ((2, 3) if osc is not None else ()):
         ref = osc.bake(params(0, order, probes[sub].copy()), mode64=True,
                        nthreads=min(16, os.cpu_count() or 1))["coeffs"]
         K = 9 if order == 2 else 16
@@ -326,7 +336,7 @@ def bench_c3(args):
             "runs": runs, "grays_per_s": round(rays / tr / 1e9, 3),
             "per_ray": {"inner_nodes": round(cnt["node_visits"] / rays, 2),
                         "triangles": round(cnt["tri_tests"] / rays, 2)},
-            "sh_rel_err_vs_fp64_by_band": errs, "error_probes": len(sub),
+            "sh_rel_err_vs_fp64_by_band": errs or None, "error_probes": len(sub) if errs else 0,
             "roofline": {"bound": "hbm", "achieved": round(alg / tr / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(alg / tr / 1e9 / HBM_PEAK_GBS, 4), "traffic": None}}
     ctx.close()
@@ -362,7 +372,7 @@ def _sun_params(vlb, probes, rays, prec, sun, lights=None):
                       shadow_bias=2e-3)
 
 
-def _err_sample(vlb, orc, oscenes, items, nprobe=8, seed=0):
+def _err_sample(vlb, oscenes, items, nprobe=8, seed=0):
     """Max rel. SH error vs the FP64 oracle over nprobe probes of each
     (scene index, params, coefficients) item."""
     import numpy as np
@@ -393,9 +403,6 @@ def bench_c4(args):
     import numpy as np
 
     import vlb
-
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import orc
 
     ctx = vlb.Context(0)
     scenes, sids, infos = [], [], []
@@ -441,13 +448,16 @@ def bench_c4(args):
     p1 = time.perf_counter()
     plain_s = p1 - p0
 
-    # error vs FP64 on 8 probes of one job per scene (and the smallest job)
-    oscenes = [orc.Scene.from_dict(sc) for sc in scenes]
-    pick = {}
-    for j, (si, pr, R) in enumerate(spec):
-        pick.setdefault(si, j)
-    items = [(spec[j][0], jobs16[j][1], got[j]) for j in sorted(pick.values())]
-    err = _err_sample(vlb, orc, oscenes, items)
+    # CPU leg: error vs FP64 on 8 probes of one job per scene
+    orc = oracle_leg(args)
+    err = None
+    if orc is not None:
+        oscenes = [orc.Scene.from_dict(sc) for sc in scenes]
+        pick = {}
+        for j, (si, pr, R) in enumerate(spec):
+            pick.setdefault(si, j)
+        items = [(spec[j][0], jobs16[j][1], got[j]) for j in sorted(pick.values())]
+        err = _err_sample(vlb, oscenes, items)
 
     rays = cnt["primary_rays"] + cnt["shadow_rays"]
     line = {"metric": METRIC, "value": round(P_all / el, 1), "unit": "probes/s", "n_gpus": 1,
@@ -486,9 +496,6 @@ def bench_c5(args):
     import numpy as np
 
     import vlb
-
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import orc
 
     ctx = vlb.Context(0)
     specs = (("city 2M", 2, 5, 0, 0),) + C4_SCENES
@@ -574,15 +581,18 @@ def bench_c5(args):
         plat.append((time.perf_counter() - q0 - arrive[i]) * 1e3)
     plain_span = time.perf_counter() - q0
 
-    # error vs FP64 on 8 probes of a few requests
-    oscenes = {}
-    items = []
-    for i in range(0, n, max(1, n // 5)):
-        si = reqs[i][0]
-        if si not in oscenes:
-            oscenes[si] = orc.Scene.from_dict(scenes[si])
-        items.append((si, ps16[i], outs[i]))
-    err = _err_sample(vlb, orc, oscenes, items)
+    # CPU leg: error vs FP64 on 8 probes of a few requests
+    orc = oracle_leg(args)
+    err = None
+    if orc is not None:
+        oscenes = {}
+        items = []
+        for i in range(0, n, max(1, n // 5)):
+            si = reqs[i][0]
+            if si not in oscenes:
+                oscenes[si] = orc.Scene.from_dict(scenes[si])
+            items.append((si, ps16[i], outs[i]))
+        err = _err_sample(vlb, oscenes, items)
 
     lat = np.asarray(lat)
     plat = np.asarray(plat)
@@ -611,6 +621,82 @@ def bench_c5(args):
     return line
 
 
+def bench_bounce(args):
+    """Multi-bounce on the c2 room: a direct bake, then bounces over the
+    plan's own 16x8x16 probe grid (irradiance lookup at every primary hit),
+    once without and once with the 8 corner visibility rays per hit.  A step
+    is one bounce (one execute, the source grid refreshed on the device)."""
+    import numpy as np
+
+    import vlb
+
+    scene = vlb.generate_scene(vlb.SCENE_ROOM, 2)
+    params = vlb.config_c2_params(scene, seed=2, precision=PREC[args.precision])
+    params.trace_variant = args.variant
+    P, R = params.probes.shape[0], params.rays
+    b0, b1, cnt3 = [0.3125, 0.25, 0.3125], [9.6875, 3.75, 9.6875], [16, 8, 16]
+    probes, step = vlb.probe_grid(b0, b1, cnt3)
+    assert np.array_equal(probes, params.probes)
+    g = vlb.grid_desc(b0, step, cnt3, num_coefs=9)
+    ctx = vlb.Context(0)
+    sid = ctx.upload_scene_dict(scene)
+    orc = oracle_leg(args)
+    oscene = orc.Scene.from_dict(scene) if orc is not None else None
+    sub = np.sort(np.random.default_rng(0).choice(P, size=32, replace=False))
+    psub = vlb.config_c2_params(scene, seed=2)
+    psub.probes = params.probes[sub].copy()
+    legs = {}
+    for vis in (False, True):
+        lp = vlb.lookup_params(vlb.LOOKUP_IRRADIANCE, vis, 2e-3)
+        plan = ctx.plan(sid, params)
+        plan.execute()
+        direct = plan.download()
+        plan.set_indirect(g, lp)
+        for _ in range(args.warmup):
+            plan.execute()
+        plan.sync()
+        plan.set_timing(True)
+        plan.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.execute()
+        plan.sync()
+        t1 = time.perf_counter()
+        st = plan.stats()
+        prev = plan.download()
+        plan.execute()
+        cur = plan.download()
+        plan.count()
+        c = plan.stats()
+        err_line = None
+        if oscene is not None:   # CPU leg: this bounce against the FP64 oracle fed the same grid
+            ref = oscene.bake_indirect(psub, g, prev, lp, 1.0, mode64=True,
+                                       nthreads=min(16, os.cpu_count() or 1))["coeffs"]
+            err = rel_err(cur[sub], ref)
+            err_line = {"max": float(err.max()), "p99": float(np.percentile(err, 99)),
+                        "probes": int(sub.size)}
+        legs[vis] = {"probes_per_s": round(P * args.steps / (t1 - t0), 1),
+                     "ms_per_bounce": round(1e3 * (t1 - t0) / args.steps, 4),
+                     "trace_ms": round(st["trace_ms"], 4),
+                     "shadow_rays_per_bounce": c["shadow_rays"],
+                     "grays_per_s": round((c["primary_rays"] + c["shadow_rays"]) * args.steps / (t1 - t0) / 1e9, 3),
+                     "c00_gain_vs_direct": round(float(np.abs(cur[:, 0, :]).sum() / np.abs(direct[:, 0, :]).sum()), 4),
+                     "sh_rel_err_vs_fp64": err_line}
+        plan.destroy()
+    ctx.close()
+    main_leg = legs[True]
+    return {
+        "metric": METRIC, "value": main_leg["probes_per_s"], "unit": "probes/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": main_leg["ms_per_bounce"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else args.precision,
+        "data": "synthetic (seeded procedural room, Fibonacci rays)",
+        "config": {"workload": "c2 multi-bounce (one step = one bounce)", "scene": "room seed 2",
+                   "probes": P, "rays_per_probe": R, "grid": cnt3, "lookup": "irradiance",
+                   "visibility_rays": True, "precision": args.precision},
+        "with_visibility": legs[True], "without_visibility": legs[False]}
+
+
 def bench_copy(args):
     """Streaming copy of 1 GiB (k_copy): attainable HBM bandwidth, and the
     reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
@@ -637,7 +723,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "copy", "sweep"])
+    ap.add_argument("--config", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "copy", "sweep", "bounce"])
     ap.add_argument("--rate", type=float, default=40.0, help="c5: request arrivals per second")
     ap.add_argument("--policy", default="srpt", choices=sorted(SCHED), help="c5: server step scheduling")
     ap.add_argument("--requests", type=int, default=200, help="c5: number of requests")
@@ -653,7 +739,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     line = {"c2": bench_c2, "c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
-            "copy": bench_copy, "sweep": sweep_c2}[args.config](args)
+            "copy": bench_copy, "sweep": sweep_c2, "bounce": bench_bounce}[args.config](args)
     print(json.dumps(line), flush=True)
 
 

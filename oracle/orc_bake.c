@@ -411,7 +411,6 @@ int orc_bake_indirect(const orc_scene* s, const vlb_bake_params* p, const vlb_pr
 {
     if (!g || !coeffs || !lp || !p || orc_check_grid(g)) return -1;
     if (p->shade_mode != VLB_SHADE_LAMBERT) return -1;
-    if ((uint64_t)g->count[0] * g->count[1] * g->count[2] != p->num_probes) return -1;
     if (g->num_coefs != (p->sh_order == 2 ? 9u : 16u)) return -1;
     return bake_common(s, p, g, coeffs, lp, scale, mode64, nthreads, brute, out, radiance, hit,
                        shadow_rays);

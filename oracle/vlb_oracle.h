@@ -99,8 +99,8 @@ int orc_bake(const orc_scene* s, const vlb_bake_params* p, int mode64, int nthre
 /* orc_bake with the multi-bounce indirect term of vlb_plan_set_indirect: at
  * each LAMBERT hit, radiance += albedo x scale x orc_probe_lookup(g, coeffs,
  * lp) at the hit point and the two-sided float shading normal; the lookup's
- * visibility rays count as shadow rays.  g must describe the bake's own
- * probes (count product = num_probes, num_coefs = K).
+ * visibility rays count as shadow rays.  g describes the source buffer
+ * (num_coefs = K); the baked probes may be any subset or other positions.
  * Pinned by: (1) zero albedo -> orc_bake exactly; (2) a constant source grid
  * (c00 only) in a closed room: result - orc_bake = albedo x scale x c00 x
  * Y00 x the projection of a constant over the same rays; (3) an l = 1 source

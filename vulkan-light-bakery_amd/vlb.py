@@ -900,6 +900,28 @@ def config_c2_params(scene: dict, seed: int = 2, precision: int = PREC_FP32) -> 
                   sun_dir=sun.tolist(), shadow_bias=1e-3)
 
 
+def procedural_cubemap(N: int, sun, zenith=2.0, horizon=0.5, sun_rgb=50.0, k=200.0,
+                       tint=(1.0, 1.0, 1.0)) -> np.ndarray:
+    """Synthetic input: 6 x N x N RGBA32F cubemap of the benchmark sky (horizon to
+    zenith gradient plus a sun lobe), Vulkan face order +X, -X, +Y, -Y, +Z, -Z."""
+    c = (np.arange(N) + 0.5) * (2.0 / N) - 1.0
+    v, u = np.meshgrid(c, c, indexing="ij")
+    one = np.ones_like(u)
+    faces = [(one, -v, -u), (-one, -v, u), (u, one, v), (u, -one, -v), (u, -v, one), (-u, -v, -one)]
+    out = np.zeros((6, N, N, 4), np.float32)
+    s = np.asarray(sun, np.float64)
+    for f, (x, y, z) in enumerate(faces):
+        l = np.sqrt(x * x + y * y + z * z)
+        x, y, z = x / l, y / l, z / l
+        g = np.maximum(y, 0.0)
+        base = horizon + (zenith - horizon) * g
+        lobe = sun_rgb * np.exp(k * (x * s[0] + y * s[1] + z * s[2] - 1.0))
+        for ch in range(3):
+            out[f, :, :, ch] = (base + lobe) * tint[ch]
+        out[f, :, :, 3] = 1.0
+    return out
+
+
 def sun_direction(rng) -> np.ndarray:
     el = rng.uniform(0.35, 1.1)
     az = rng.uniform(0.0, 2 * np.pi)
