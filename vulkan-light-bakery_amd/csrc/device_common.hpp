@@ -341,6 +341,27 @@ __device__ __forceinline__ bool next_node(float e[4], int c[4], const Stack& st,
     cswap(e[1], c[1], e[3], c[3]);
     cswap(e[1], c[1], e[2], c[2]);
     const float inf = __builtin_huge_valf();
+    if (__all(sp + 3 <= kLdsDepth)) {
+        // Every active lane has three free LDS entries (one wave-uniform
+        // test): straight-line pushes with conditional increments.  A write
+        // past the top lands at most where the all-hit case writes, and is
+        // overwritten by the next push or left unused.  Hit entries are
+        // sorted first, so the pushes are the same as below.
+        st.lds[sp * 256] = (uint32_t)c[3];
+        sp += e[3] != inf ? 1 : 0;
+        st.lds[sp * 256] = (uint32_t)c[2];
+        sp += e[2] != inf ? 1 : 0;
+        st.lds[sp * 256] = (uint32_t)c[1];
+        sp += e[1] != inf ? 1 : 0;
+        if (e[0] != inf) {
+            node = (uint32_t)c[0];
+            return true;
+        }
+        if (sp == 0) return false;
+        --sp;
+        node = st.lds[sp * 256];
+        return true;
+    }
     if (e[3] != inf) push(st, sp, (uint32_t)c[3]);
     if (e[2] != inf) push(st, sp, (uint32_t)c[2]);
     if (e[1] != inf) push(st, sp, (uint32_t)c[1]);

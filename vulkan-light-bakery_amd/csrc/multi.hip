@@ -141,10 +141,13 @@ __global__ __launch_bounds__(256) void k_rays_jobs(MultiArgs a, const JobDesc* _
                                                    const WorkHeader* __restrict__ hdr)
 {
     __shared__ uint32_t lds[kLdsDepth * 256];
+    __shared__ uint32_t qlds[kLeafQueue * 256];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     Stack st;
     st.lds = &lds[tid];
+    LeafQueue q;
+    q.lds = &qlds[tid];
     st.spill = a.spill + (size_t)blockIdx.x * 256 + tid;
     st.spill_stride = a.spill_stride;
     TraceCount cnt;
@@ -222,7 +225,7 @@ __global__ __launch_bounds__(256) void k_rays_jobs(MultiArgs a, const JobDesc* _
         }
         // --- one traversal step for active lanes
         if (act) {
-            if (trav_step<ANY, COUNT>(R.T, R.nodes, R.tris, st, cnt)) {
+            if (trav_step<ANY, COUNT>(R.T, R.nodes, R.tris, st, q, cnt)) {
                 if (ANY) {
                     a.vis[(size_t)R.slot * a.nvis + R.light] = (uint8_t)(R.T.best == 0 ? 1 : 0);
                 } else if (R.T.best >= 0) {

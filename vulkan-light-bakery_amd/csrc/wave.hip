@@ -126,10 +126,13 @@ template <bool ANY, bool COUNT, bool CORNERS>
 __global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigned long long* ctr)
 {
     __shared__ uint32_t lds[kLdsDepth * 256];
+    __shared__ uint32_t qlds[kLeafQueue * 256];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     Stack st;
     st.lds = &lds[tid];
+    LeafQueue q;
+    q.lds = &qlds[tid];
     st.spill = a.spill + (size_t)blockIdx.x * 256 + tid;
     st.spill_stride = a.spill_stride;
     TraceCount cnt;
@@ -181,7 +184,7 @@ __global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigne
         }
         // --- one traversal step for active lanes
         if (act) {
-            if (trav_step<ANY, COUNT>(T, a.sc.nodes, a.sc.tris, st, cnt)) {
+            if (trav_step<ANY, COUNT>(T, a.sc.nodes, a.sc.tris, st, q, cnt)) {
                 if (ANY) {
                     a.vis[(size_t)slot * a.nvis + light] = (uint8_t)(T.best == 0 ? 1 : 0);
                 } else if (T.best >= 0) {

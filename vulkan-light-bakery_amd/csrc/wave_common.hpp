@@ -106,6 +106,21 @@ __device__ __forceinline__ ShadowRay shadow_ray(const ShadeParams& sp, const Hit
 }
 
 // ------------------------------------------------------------ traversal
+// Postponed leaf tests (Aila and Laine, "Understanding the Efficiency of Ray
+// Traversal on GPUs", HPG 2009): a lane that hits leaf children queues their
+// triangles and keeps visiting inner nodes; the wave tests the queues
+// together once every active lane has queued triangles or no nodes left, or
+// when a queue could overflow.  Testing leaves at every visit instead runs
+// each triangle test with the few lanes that happened to hit a leaf.  A
+// postponed test means some nodes are culled against an older tbest, which
+// only adds visits: the closest hit is still the (t, prim id) minimum over a
+// superset of the candidates, and any-hit still reports whether one exists.
+constexpr int kLeafQueue = 8;  // entries per lane, in LDS
+
+struct LeafQueue {
+    uint32_t* lds;  // entry i of this lane at lds[i * 256]
+};
+
 struct TravState {
     WtRay w;
     SlabRay s;
@@ -114,6 +129,8 @@ struct TravState {
     float bV, bW, bdet;
     uint32_t node;
     int sp;
+    int nq;     // queued triangles
+    bool live;  // `node` is still to be visited
 };
 
 __device__ __forceinline__ void trav_init(TravState& T, float maxabs, const float3& o,
@@ -129,42 +146,74 @@ __device__ __forceinline__ void trav_init(TravState& T, float maxabs, const floa
     T.bdet = 1.0f;
     T.node = 0;
     T.sp = 0;
+    T.nq = 0;
+    T.live = true;
 }
 
-// One node visit.  Returns true when the ray is finished (ANY: on the first
-// accepted triangle, with T.best = 0).
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ bool trav_step(TravState& T, const BNode4* nodes, const float4* tris,
-                                          const Stack& st, TraceCount& cnt)
+// One inner-node visit: hit leaf children are appended to the queue (four
+// straight-line writes with conditional increments; needs nq <= kLeafQueue
+// - 4, so a write past the end stays inside the queue), hit inner children
+// go on the stack.
+template <bool COUNT>
+__device__ __forceinline__ void visit_node(TravState& T, const BNode4* nodes, const Stack& st,
+                                           const LeafQueue& q, TraceCount& cnt)
 {
     const BNode4 n = load_node(nodes, T.node);
     if (COUNT) ++cnt.nodes;
     float e[4];
     int c[4];
     slab4(T.s, n, T.tmin, T.tb, e, c);
-    // leaves one at a time per lane (the wave runs as many tests as its
-    // busiest lane has hit leaves)
-    for (uint32_t leaves = take_leaves(e, c); leaves != 0; leaves &= leaves - 1) {
-        float3 v0, v1, v2;
-        int prim;
-        load_tri(tris, ~pick4(c, __builtin_ctz(leaves)), v0, v1, v2, prim);
-        if (COUNT) ++cnt.tris;
-        WtHit h;
-        if (ANY) {
-            if (wt_test(T.w, v0, v1, v2, T.tmin, T.tb, h)) {
-                T.best = 0;
-                return true;
+    const uint32_t leaves = take_leaves(e, c);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        q.lds[T.nq * 256] = (uint32_t)~c[k];
+        T.nq += (int)((leaves >> k) & 1u);
+    }
+    T.live = next_node(e, c, st, T.sp, T.node);
+}
+
+// Tests every queued triangle of the active lanes (the wave runs as many
+// rounds as its longest queue) and empties the queues.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ void test_queue(TravState& T, const float4* tris, const LeafQueue& q,
+                                           TraceCount& cnt)
+{
+    for (int i = 0; __any(i < T.nq); ++i) {
+        if (i < T.nq && !(ANY && T.best == 0)) {
+            float3 v0, v1, v2;
+            int prim;
+            load_tri(tris, (int)q.lds[i * 256], v0, v1, v2, prim);
+            if (COUNT) ++cnt.tris;
+            WtHit h;
+            if (ANY) {
+                if (wt_test(T.w, v0, v1, v2, T.tmin, T.tb, h)) T.best = 0;
+            } else if (wt_test(T.w, v0, v1, v2, T.tmin, T.tb, h) &&
+                       (T.best < 0 || h.t < T.tb || prim < T.best)) {
+                // t <= tbest holds on acceptance; an equal t keeps the lower prim id
+                T.tb = h.t;
+                T.best = prim;
+                T.bV = h.V;
+                T.bW = h.W;
+                T.bdet = h.det;
             }
-        } else if (wt_test(T.w, v0, v1, v2, T.tmin, T.tb, h) &&
-                   (T.best < 0 || h.t < T.tb || prim < T.best)) {
-            T.tb = h.t;
-            T.best = prim;
-            T.bV = h.V;
-            T.bW = h.W;
-            T.bdet = h.det;
         }
     }
-    return !next_node(e, c, st, T.sp, T.node);
+    T.nq = 0;
+    if (ANY && T.best == 0) T.live = false;
+}
+
+// One step of a lane's ray, called by the active lanes of a wave together:
+// a node visit, then (wave-uniform) the queued tests when due.  Returns true
+// when the ray is finished (ANY: T.best = 0 on an accepted triangle).
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool trav_step(TravState& T, const BNode4* nodes, const float4* tris,
+                                          const Stack& st, const LeafQueue& q, TraceCount& cnt)
+{
+    if (T.live && T.nq <= kLeafQueue - 4) visit_node<COUNT>(T, nodes, st, q, cnt);
+    const bool ready = T.nq > 0 || !T.live;
+    if ((__all(ready) && __any(T.nq > 0)) || __any(T.nq > kLeafQueue - 4))
+        test_queue<ANY, COUNT>(T, tris, q, cnt);
+    return !T.live && T.nq == 0;
 }
 
 // Sky radiance of a missed primary ray, with the REFERENCE conversions.
