@@ -92,7 +92,13 @@ void orc_trace_occluded_batch(const orc_scene* s, const float* org, const float*
  * Projection always accumulates in double.  out: P*K*3 doubles.
  * radiance (P*R*3) and hit (P*R) are optional per-ray outputs in canonical
  * ray order.  nthreads <= 0: one thread per online core.
- * Returns 0 on success, -1 on bad parameters. */
+ * Returns 0 on success, -1 on bad parameters.
+ * Pinned by (tests/test_oracle_bake.py, closed forms over one plane): sky-only
+ * bakes = the quadrature sum_r (4 pi / R) Y_k L_r with the scipy-pinned basis;
+ * Lambert with a directional light = albedo E cos / pi on every floor hit
+ * (both windings: the two-sided normal); point light = albedo I cos /
+ * (pi d^2) per analytic hit point, with analytic shadows of a square
+ * occluder; REFERENCE = albedo (ambient + Cd sDotN + Cs max(R.d, 0)^gloss). */
 int orc_bake(const orc_scene* s, const vlb_bake_params* p, int mode64, int nthreads,
              int brute, double* out, float* radiance, int32_t* hit,
              uint64_t* shadow_rays);
