@@ -283,7 +283,7 @@ def test_plan_graph_replays_match_eager(ctx, room):
 
 # ---- every trace pipeline computes the same bits: fused k_trace (1..12) and
 # the wavefront k_rays/k_shade pipeline (13..24)
-VARIANTS = [1, 3, 4, 8, 13, 15, 16, 20, 23]
+VARIANTS = [1, 3, 4, 8, 12, 13, 15, 16, 20, 23]
 
 
 def test_trace_variants_are_bitwise_identical(ctx, room):

@@ -751,9 +751,9 @@ vlb_status vlb_plan_autotune(vlb_plan* h, uint32_t iters, uint32_t* chosen)
         hipEvent_t e0, e1;
         VLB_HIP(hipEventCreate(&e0));
         VLB_HIP(hipEventCreate(&e1));
-        // fused: lane maps 0..3 unbounded + maps 2, 3 at 6 waves/SIMD;
+        // fused: lane maps 0..3 unbounded + maps 2, 3 at 6 and 8 waves/SIMD;
         // wavefront: maps 0..3 with refill 16, maps 2, 3 with refill 32 / 8
-        const uint32_t cand[] = {1, 2, 3, 4, 7, 8, 13, 14, 15, 16, 19, 20, 23, 24};
+        const uint32_t cand[] = {1, 2, 3, 4, 7, 8, 11, 12, 13, 14, 15, 16, 19, 20, 23, 24};
         double best = 1e30;
         uint32_t best_v = pl->variant;
         for (uint32_t v : cand) {
