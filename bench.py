@@ -216,6 +216,7 @@ def bench_c1(args):
     orc = oracle_leg(args)
     err = rel_err(out[0], orc.skybox_sh(base[0], 9)).max() if orc is not None else None
     kms = float(np.mean(proj))
+    traffic = load_traffic("skybox_c1_traffic.json")
     achieved = (nbytes + B * 27 * 4) / (kms / 1e3) / 1e9
     line = {"metric": "skyboxes/s (6x128x128 RGBA32F -> L2)", "value": round(B * args.steps / (t1 - t0), 1),
             "unit": "skyboxes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -224,7 +225,8 @@ def bench_c1(args):
             "config": {"workload": "c1 skybox SH", "batch": B, "face": N},
             "sh_rel_err_vs_fp64": None if err is None else float(err),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None},
             "cpu_baseline": None}
     ctx.close()
     return line

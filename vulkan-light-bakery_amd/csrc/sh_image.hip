@@ -53,10 +53,48 @@ __device__ __forceinline__ float3 cube_dir(int f, float u, float v)
     case 4: d = make_float3(u, -v, 1.0f); break;
     default: d = make_float3(-u, -v, -1.0f); break;
     }
-    const float inv = 1.0f / sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    // hardware reciprocal square root (~1 ulp): the projection is checked to
+    // 1e-5 relative against the FP64 oracle, and this is ~20 instructions
+    // cheaper per texel than a correctly rounded divide of a sqrt
+    const float inv = rsqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
     return make_float3(d.x * inv, d.y * inv, d.z * inv);
 }
 
+// One texel: SH basis at its direction times its solid angle, into acc.
+template <int K>
+__device__ __forceinline__ void skybox_texel(float* acc, const float4& px, int t, int N, int logN,
+                                             int NN, float s2, float s1,
+                                             const float* __restrict__ wtab)
+{
+    int f, rem, j, i;
+    if (logN >= 0) {
+        f = t >> (2 * logN);
+        rem = t & (NN - 1);
+        j = rem >> logN;
+        i = rem & (N - 1);
+    } else {
+        f = t / NN;
+        rem = t - f * NN;
+        j = rem / N;
+        i = rem - j * N;
+    }
+    const float u = fmaf((float)i, s2, s1);
+    const float v = fmaf((float)j, s2, s1);
+    const float3 d = cube_dir(f, u, v);
+    const float w = wtab[rem];
+    const float r = px.x * w, g = px.y * w, b = px.z * w;
+    float Y[16];
+    sh_exact<K>(d.x, d.z, d.y, Y);  // SH frame: world (x, z, y)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        acc[3 * k + 0] = fmaf(Y[k], r, acc[3 * k + 0]);
+        acc[3 * k + 1] = fmaf(Y[k], g, acc[3 * k + 1]);
+        acc[3 * k + 2] = fmaf(Y[k], b, acc[3 * k + 2]);
+    }
+}
+
+// Four texels per thread per round, their loads issued before any math so
+// that each lane keeps 64 B in flight.
 template <int K>
 __global__ __launch_bounds__(256) void k_skybox(const float4* __restrict__ cubes, int N, int logN,
                                                 int per_block, int blocks_per_cube,
@@ -75,34 +113,21 @@ __global__ __launch_bounds__(256) void k_skybox(const float4* __restrict__ cubes
     float acc[K * 3];
 #pragma unroll
     for (int k = 0; k < K * 3; ++k) acc[k] = 0.0f;
-    for (int t = start + (int)threadIdx.x; t < end; t += 256) {
-        typedef float nf4 __attribute__((ext_vector_type(4)));
-        const nf4 px = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(src + t));
-        int f, rem, j, i;
-        if (logN >= 0) {
-            f = t >> (2 * logN);
-            rem = t & (NN - 1);
-            j = rem >> logN;
-            i = rem & (N - 1);
-        } else {
-            f = t / NN;
-            rem = t - f * NN;
-            j = rem / N;
-            i = rem - j * N;
-        }
-        const float u = fmaf((float)i, s2, s1);
-        const float v = fmaf((float)j, s2, s1);
-        const float3 d = cube_dir(f, u, v);
-        const float w = wtab[rem];
-        const float r = px[0] * w, g = px[1] * w, b = px[2] * w;
-        float Y[16];
-        sh_exact<K>(d.x, d.z, d.y, Y);  // SH frame: world (x, z, y)
+    typedef float nf4 __attribute__((ext_vector_type(4)));
+    int t = start + (int)threadIdx.x;
+    for (; t + 768 < end; t += 1024) {
+        nf4 q[4];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            acc[3 * k + 0] = fmaf(Y[k], r, acc[3 * k + 0]);
-            acc[3 * k + 1] = fmaf(Y[k], g, acc[3 * k + 1]);
-            acc[3 * k + 2] = fmaf(Y[k], b, acc[3 * k + 2]);
-        }
+        for (int u = 0; u < 4; ++u)
+            q[u] = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(src + t + 256 * u));
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            skybox_texel<K>(acc, make_float4(q[u][0], q[u][1], q[u][2], q[u][3]), t + 256 * u, N, logN,
+                            NN, s2, s1, wtab);
+    }
+    for (; t < end; t += 256) {
+        const nf4 q = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(src + t));
+        skybox_texel<K>(acc, make_float4(q[0], q[1], q[2], q[3]), t, N, logN, NN, s2, s1, wtab);
     }
     block_reduce<K * 3>(acc, lds);
     if (threadIdx.x == 0)
