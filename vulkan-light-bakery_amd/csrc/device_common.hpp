@@ -514,6 +514,44 @@ __device__ float3 grid_indirect(const SceneView& sc, const ProbeGridView& g, con
     return make_float3(g.scale * (acc[0] / sw), g.scale * (acc[1] / sw), g.scale * (acc[2] / sw));
 }
 
+// Lambert term of light j at hit point P with the two-sided normal N: the
+// contribution E cos is formed before the shadow query, so only it (not E
+// and the cosine) is live across the traversal; added to acc when unoccluded.
+template <bool COUNT>
+__device__ __forceinline__ void light_term(const SceneView& sc, const ShadeParams& sp, int j,
+                                           const float3& P, const float3& N, float3& acc,
+                                           const Stack& st, TraceCount& cnt, uint32_t& nshadow)
+{
+    const vlb_light& Lj = sp.lights[j];
+    float3 Ld;
+    float dist;
+    float3 E;
+    if (Lj.type == VLB_LIGHT_POINT) {
+        const float3 Lv = make_float3(Lj.vec[0] - P.x, Lj.vec[1] - P.y, Lj.vec[2] - P.z);
+        dist = sqrtf(dot3(Lv, Lv));
+        Ld = dist > 0.0f ? make_float3(Lv.x / dist, Lv.y / dist, Lv.z / dist)
+                         : make_float3(0.f, 0.f, 0.f);
+        const float d2 = dist * dist;
+        E = make_float3(Lj.rgb[0] / d2, Lj.rgb[1] / d2, Lj.rgb[2] / d2);
+    } else {
+        Ld = make_float3(-Lj.vec[0], -Lj.vec[1], -Lj.vec[2]);
+        dist = sp.t_max;
+        E = make_float3(Lj.rgb[0], Lj.rgb[1], Lj.rgb[2]);
+    }
+    const float cosNL = dot3(N, Ld);
+    if (cosNL > 0.0f) {
+        const float3 c = make_float3(E.x * cosNL, E.y * cosNL, E.z * cosNL);
+        const float3 org = make_float3(P.x + sp.shadow_bias * N.x, P.y + sp.shadow_bias * N.y,
+                                       P.z + sp.shadow_bias * N.z);
+        ++nshadow;
+        if (!trace_any<COUNT>(sc, org, Ld, 0.0f, dist, st, cnt)) {
+            acc.x = acc.x + c.x;
+            acc.y = acc.y + c.y;
+            acc.z = acc.z + c.z;
+        }
+    }
+}
+
 // Radiance carried by one primary ray.  `nshadow` counts traced shadow rays.
 // IND compiles the multi-bounce term in (used only when sp.ind.coeffs is
 // set); direct-only kernels are built without it and keep its registers free.
@@ -554,8 +592,8 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
     m.z = s0.z * b0 + s1.z * u + s2.z * v;
     const float ml = sqrtf(dot3(m, m));
     float3 N = ml > 0.0f ? make_float3(m.x / ml, m.y / ml, m.z / ml) : make_float3(0.f, 0.f, 0.f);
-    const float3 alb = make_float3(s0.w, s1.w, s2.w);
     if (ref) {
+        const float3 alb = make_float3(s0.w, s1.w, s2.w);
         // env_map.rchit:73-101, main.rchit:95-116
         const vlb_ref_shading& r = sp.ref;
         const float3 L = make_float3(r.light_position[0] - P.x, r.light_position[1] - P.y,
@@ -601,35 +639,15 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
     // LAMBERT, two-sided
     if (dot3(N, d) > 0.0f) N = make_float3(-N.x, -N.y, -N.z);
     float3 acc = make_float3(0.f, 0.f, 0.f);
-    for (int j = 0; j < sp.nlights; ++j) {
-        const vlb_light& Lj = sp.lights[j];
-        float3 Ld;
-        float dist;
-        float3 E;
-        if (Lj.type == VLB_LIGHT_POINT) {
-            const float3 Lv = make_float3(Lj.vec[0] - P.x, Lj.vec[1] - P.y, Lj.vec[2] - P.z);
-            dist = sqrtf(dot3(Lv, Lv));
-            Ld = dist > 0.0f ? make_float3(Lv.x / dist, Lv.y / dist, Lv.z / dist)
-                             : make_float3(0.f, 0.f, 0.f);
-            const float d2 = dist * dist;
-            E = make_float3(Lj.rgb[0] / d2, Lj.rgb[1] / d2, Lj.rgb[2] / d2);
-        } else {
-            Ld = make_float3(-Lj.vec[0], -Lj.vec[1], -Lj.vec[2]);
-            dist = sp.t_max;
-            E = make_float3(Lj.rgb[0], Lj.rgb[1], Lj.rgb[2]);
-        }
-        const float cosNL = dot3(N, Ld);
-        if (cosNL > 0.0f) {
-            const float3 org = make_float3(P.x + sp.shadow_bias * N.x, P.y + sp.shadow_bias * N.y,
-                                           P.z + sp.shadow_bias * N.z);
-            ++nshadow;
-            if (!trace_any<COUNT>(sc, org, Ld, 0.0f, dist, st, cnt)) {
-                acc.x = acc.x + E.x * cosNL;
-                acc.y = acc.y + E.y * cosNL;
-                acc.z = acc.z + E.z * cosNL;
-            }
-        }
-    }
+    // the last light is peeled off the loop: after its shadow ray is set up,
+    // neither the hit point nor the normal is live across its traversal
+    // (unless the indirect term needs them)
+    const int nl = sp.nlights;
+    for (int j = 0; j + 1 < nl; ++j) light_term<COUNT>(sc, sp, j, P, N, acc, st, cnt, nshadow);
+    if (nl > 0) light_term<COUNT>(sc, sp, nl - 1, P, N, acc, st, cnt, nshadow);
+    // albedo read after the shadow queries rather than held across them
+    const float3 alb = make_float3(sc.shade[3 * prim + 0].w, sc.shade[3 * prim + 1].w,
+                                   sc.shade[3 * prim + 2].w);
     const float kInvPi = 0.318309886183790671538f;
     if (!IND || !sp.ind.coeffs)
         return make_float3(alb.x * acc.x * kInvPi, alb.y * acc.y * kInvPi, alb.z * acc.z * kInvPi);
