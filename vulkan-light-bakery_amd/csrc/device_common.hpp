@@ -593,7 +593,6 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
     const float ml = sqrtf(dot3(m, m));
     float3 N = ml > 0.0f ? make_float3(m.x / ml, m.y / ml, m.z / ml) : make_float3(0.f, 0.f, 0.f);
     if (ref) {
-        const float3 alb = make_float3(s0.w, s1.w, s2.w);
         // env_map.rchit:73-101, main.rchit:95-116
         const vlb_ref_shading& r = sp.ref;
         const float3 L = make_float3(r.light_position[0] - P.x, r.light_position[1] - P.y,
@@ -619,6 +618,10 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
             const float rd = dot3(R, d);
             spec = r.c_specular * powf(rd > 0.0f ? rd : 0.0f, r.glossiness);
         }
+        // albedo read after the shadow query (see the Lambert path below)
+        __asm__ __volatile__("" ::: "memory");
+        const float3 alb = make_float3(sc.shade[3 * prim + 0].w, sc.shade[3 * prim + 1].w,
+                                       sc.shade[3 * prim + 2].w);
         const bool zero = alb.x == 0.0f && alb.y == 0.0f && alb.z == 0.0f;
         float3 c;
         c.x = (zero ? 1.0f : alb.x) * (r.ambient + diffuse + spec);
@@ -645,7 +648,10 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
     const int nl = sp.nlights;
     for (int j = 0; j + 1 < nl; ++j) light_term<COUNT>(sc, sp, j, P, N, acc, st, cnt, nshadow);
     if (nl > 0) light_term<COUNT>(sc, sp, nl - 1, P, N, acc, st, cnt, nshadow);
-    // albedo read after the shadow queries rather than held across them
+    // albedo read after the shadow queries rather than held across them (the
+    // compiler barrier keeps it from reusing the loads of the normals, which
+    // it would otherwise keep in scratch across the traversals)
+    __asm__ __volatile__("" ::: "memory");
     const float3 alb = make_float3(sc.shade[3 * prim + 0].w, sc.shade[3 * prim + 1].w,
                                    sc.shade[3 * prim + 2].w);
     const float kInvPi = 0.318309886183790671538f;
