@@ -213,8 +213,27 @@ def bench_c1(args):
         proj.append(ctx.skybox_last_ms()[0])
     out = np.zeros((B, 9, 3), np.float32)
     ctx.download(d_out, out)
+    # batch = 1: one skybox per call, synchronised (launch-latency bound)
+    lat, k1 = [], []
+    for i in range(50):
+        c0 = time.perf_counter()
+        ctx.skybox_sh_device(d_in, N, 1, 2, d_out)
+        ctx.sync()
+        lat.append(time.perf_counter() - c0)
+        k1.append(ctx.skybox_last_ms()[0])
     orc = oracle_leg(args)
-    err = rel_err(out[0], orc.skybox_sh(base[0], 9)).max() if orc is not None else None
+    err, cpu = None, None
+    if orc is not None:
+        err = rel_err(out[0], orc.skybox_sh(base[0], 9)).max()
+        # CPU leg: the oracle's projection (one thread, FP64) of the same cubemaps
+        n_cpu, c0 = 0, time.perf_counter()
+        while n_cpu < 8 or time.perf_counter() - c0 < 5.0:
+            orc.skybox_sh(base[n_cpu % 8], 9)
+            n_cpu += 1
+        c1 = time.perf_counter()
+        cpu = {"value": round(n_cpu / (c1 - c0), 1), "unit": "skyboxes/s", "cores": 1, "kind": "port",
+               "sample": f"C oracle (FP64, exact solid angles), {n_cpu} of the same 6x128x128 cubemaps "
+                         f"projected one after another on one thread, {c1 - c0:.1f} s"}
     kms = float(np.mean(proj))
     traffic = load_traffic("skybox_c1_traffic.json")
     achieved = (nbytes + B * 27 * 4) / (kms / 1e3) / 1e9
@@ -224,10 +243,12 @@ def bench_c1(args):
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "c1 skybox SH", "batch": B, "face": N},
             "sh_rel_err_vs_fp64": None if err is None else float(err),
+            "batch1": {"wall_us_per_skybox": round(1e6 * float(np.median(lat)), 1),
+                       "kernel_us": round(1e3 * float(np.median(k1)), 2), "calls": len(lat)},
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None},
-            "cpu_baseline": None}
+            "cpu_baseline": cpu}
     ctx.close()
     return line
 
@@ -312,6 +333,14 @@ def bench_c3(args):
     sub = np.sort(rng.choice(len(probes), size=24, replace=False))
     orc = oracle_leg(args)
     osc = orc.Scene.from_dict(scene) if orc is not None else None
+    cpu = None
+    if osc is not None:
+        # CPU leg: the FP32 oracle on a fixed seeded subset of 256 probes, L3
+        spp, n = _cpu_seconds_per_probe(vlb, osc, params(0, 3), 256, 5.0, seed=3)
+        cpu = {"value": round(1.0 / spp, 2), "unit": "probes/s", "cores": CPU_THREADS, "kind": "port",
+               "sample": f"FP32 C oracle, a fixed seeded subset of 256 of the 16,384 probes baked repeatedly "
+                         f"({n} probe bakes of 16,384 rays, L3, same scene, BVH-independent results), "
+                         f"{CPU_THREADS} threads, {spp * n:.1f} s"}
     errs = {}
     for order in ((2, 3) if osc is not None else ()):
         ref = osc.bake(params(0, order, probes[sub].copy()), mode64=True,
@@ -339,6 +368,7 @@ def bench_c3(args):
             "per_ray": {"inner_nodes": round(cnt["node_visits"] / rays, 2),
                         "triangles": round(cnt["tri_tests"] / rays, 2)},
             "sh_rel_err_vs_fp64_by_band": errs or None, "error_probes": len(sub) if errs else 0,
+            "cpu_baseline": cpu,
             "roofline": {"bound": "hbm", "achieved": round(alg / tr / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(alg / tr / 1e9 / HBM_PEAK_GBS, 4), "traffic": None}}
     ctx.close()
@@ -372,6 +402,31 @@ def _sun_params(vlb, probes, rays, prec, sun, lights=None):
                       lights=lights if lights is not None else
                       [(vlb.LIGHT_DIRECTIONAL, (-sun).tolist(), (3.0, 3.0, 3.0))],
                       shadow_bias=2e-3)
+
+
+CPU_THREADS = min(16, os.cpu_count() or 1)
+
+
+def _cpu_seconds_per_probe(vlb, oscene, prm, nprobe, min_seconds, seed=0):
+    """Wall seconds per probe of the FP32 CPU oracle (CPU_THREADS threads) on
+    a seeded subset of the job's probes (per-probe cost is uniform in the
+    ray count), baked repeatedly until min_seconds have passed; and the
+    number of probes baked."""
+    import numpy as np
+
+    P = prm.probes.shape[0]
+    rng = np.random.default_rng(seed)
+    sub = np.sort(rng.choice(P, size=min(nprobe, P), replace=False))
+    q = vlb.Params(**{**prm.__dict__, "probes": prm.probes[sub].copy(), "_keep": []})
+    q.precision = 0
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oscene.bake(q, mode64=False, nthreads=CPU_THREADS)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= min_seconds:
+            break
+    return el / (reps * sub.size), int(reps * sub.size)
 
 
 def _err_sample(vlb, oscenes, items, nprobe=8, seed=0):
@@ -452,7 +507,7 @@ def bench_c4(args):
 
     # CPU leg: error vs FP64 on 8 probes of one job per scene
     orc = oracle_leg(args)
-    err = None
+    err, cpu = None, None
     if orc is not None:
         oscenes = [orc.Scene.from_dict(sc) for sc in scenes]
         pick = {}
@@ -460,6 +515,16 @@ def bench_c4(args):
             pick.setdefault(si, j)
         items = [(spec[j][0], jobs16[j][1], got[j]) for j in sorted(pick.values())]
         err = _err_sample(vlb, oscenes, items)
+        # CPU leg: every job on the FP32 oracle, 16 seeded probes each, scaled
+        # to the job's probe count (per-probe cost is uniform within a job)
+        est, t_cpu = 0.0, 0.0
+        for j, (si, pr, R) in enumerate(spec):
+            spp, n = _cpu_seconds_per_probe(vlb, oscenes[si], jobs32[j][1], 16, 0.15, seed=j)
+            est += spp * pr.shape[0]
+            t_cpu += spp * n
+        cpu = {"value": round(P_all / est, 2), "unit": "probes/s", "cores": CPU_THREADS, "kind": "port",
+               "sample": f"FP32 C oracle, the same 64 jobs serially, 16 seeded probes of each baked repeatedly "
+                         f"for >= 0.15 s and scaled to the job's size, {CPU_THREADS} threads, {t_cpu:.1f} s"}
 
     rays = cnt["primary_rays"] + cnt["shadow_rays"]
     line = {"metric": METRIC, "value": round(P_all / el, 1), "unit": "probes/s", "n_gpus": 1,
@@ -479,6 +544,7 @@ def bench_c4(args):
                                  "calls": len(jobs32)},
             "speedup_vs_plain_fp32": round(plain_s / el, 3),
             "sh_rel_err_vs_fp64": err,
+            "cpu_baseline": cpu,
             "device_bytes": info["device_bytes"]}
     b.destroy()
     ctx.close()
@@ -583,9 +649,11 @@ def bench_c5(args):
         plat.append((time.perf_counter() - q0 - arrive[i]) * 1e3)
     plain_span = time.perf_counter() - q0
 
-    # CPU leg: error vs FP64 on 8 probes of a few requests
+    # CPU leg: error vs FP64 on 8 probes of a few requests, and the FP32
+    # oracle's rate on the request mix (16 probes of one request per scene,
+    # scaled to every request of that scene)
     orc = oracle_leg(args)
-    err = None
+    err, cpu = None, None
     if orc is not None:
         oscenes = {}
         items = []
@@ -595,6 +663,21 @@ def bench_c5(args):
                 oscenes[si] = orc.Scene.from_dict(scenes[si])
             items.append((si, ps16[i], outs[i]))
         err = _err_sample(vlb, oscenes, items)
+        spp_scene, t_cpu = {}, 0.0
+        for i in range(n):
+            si = reqs[i][0]
+            if si in spp_scene:
+                continue
+            if si not in oscenes:
+                oscenes[si] = orc.Scene.from_dict(scenes[si])
+            spp, m = _cpu_seconds_per_probe(vlb, oscenes[si], ps32[i], 16, 2.0, seed=i)
+            spp_scene[si] = spp
+            t_cpu += spp * m
+        est = sum(spp_scene[r[0]] * r[1].shape[0] for r in reqs)
+        cpu = {"value": round(P_all / est, 2), "unit": "probes/s", "cores": CPU_THREADS, "kind": "port",
+               "sample": f"FP32 C oracle, 16 probes of one request per scene ({len(spp_scene)} scenes) baked "
+                         f"repeatedly for >= 2 s each and scaled to all {n} requests, {CPU_THREADS} threads, "
+                         f"{t_cpu:.1f} s"}
 
     lat = np.asarray(lat)
     plat = np.asarray(plat)
@@ -618,7 +701,8 @@ def bench_c5(args):
                                        "p50_ms": round(float(np.percentile(plat, 50)), 2),
                                        "p99_ms": round(float(np.percentile(plat, 99)), 2),
                                        "span_s": round(plain_span, 3)},
-            "sh_rel_err_vs_fp64": err}
+            "sh_rel_err_vs_fp64": err,
+            "cpu_baseline": cpu}
     ctx.close()
     return line
 
