@@ -601,7 +601,8 @@ def bench_c5(args):
 
     # ---- the server
     srv = vlb.Server(ctx, precision=PREC["bf16"], state_budget=args.state_budget << 20,
-                     step_rays=args.step_rays, policy=SCHED[args.policy])
+                     step_rays=args.step_rays, policy=SCHED[args.policy],
+                     trace_variant=args.variant)
     # warm-up: one small request per scene (direction tables, graph capture)
     for si in range(len(scenes)):
         w = (si, reqs[0][1][:256], sun0, [(vlb.LIGHT_DIRECTIONAL, (-sun0).tolist(), (3.0, 3.0, 3.0))])
@@ -690,7 +691,7 @@ def bench_c5(args):
                        "scenes": [{"scene": specs[i][0], "triangles": infos[i]["num_tris"],
                                    "bvh_depth": infos[i]["depth"]} for i in range(len(specs))],
                        "state_budget_mib": args.state_budget, "step_rays": args.step_rays,
-                       "policy": args.policy},
+                       "policy": args.policy, "trace_variant": args.variant or 12},
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
                            "p99": round(float(np.percentile(lat, 99)), 2),
                            "max": round(float(lat.max()), 2)},

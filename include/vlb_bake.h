@@ -171,7 +171,7 @@ typedef struct vlb_bake_params {
                                       /*   shadow, shade): 13 + map +        */
                                       /*   4*refill, refill {16, 32, 8}      */
                                       /*   idle lanes; 0 = autotuned choice, */
-                                      /*   else 20                           */
+                                      /*   else 12 (20 with an indirect term)*/
     uint32_t         reserved[2];
 } vlb_bake_params;
 
@@ -209,6 +209,9 @@ int32_t     vlb_abi_version(void);
  * selection, queues and pools become one HIP device + one stream.
  * hbm_budget_bytes == 0 means "all free memory". */
 vlb_status  vlb_ctx_create(int32_t device, uint64_t hbm_budget_bytes, vlb_ctx** out);
+/* Frees the context's scenes and device buffers.  VLB_ERR_INVALID (and
+ * nothing freed) while plans, batches or servers made from it are alive:
+ * destroy those first. */
 vlb_status  vlb_ctx_destroy(vlb_ctx* ctx);
 const char* vlb_last_error(const vlb_ctx* ctx);
 /* Number of compute units and device name (for reports). */
@@ -305,7 +308,9 @@ typedef struct vlb_batch_job {
 } vlb_batch_job;
 typedef struct vlb_batch_options {
     uint64_t max_round_rays;  /* 0 = 2^28; at most 2^30                   */
-    uint32_t trace_variant;   /* 13..24 wavefront lane map x refill; 0=20 */
+    uint32_t trace_variant;   /* 1..12 fused kernel (map x waves/SIMD as */
+                              /* in vlb_bake_params), 13..24 wavefront  */
+                              /* lane map x refill; 0 = 12               */
     uint32_t chunk;           /* rays per work fetch: 64..1024, divides   */
                               /* 1024; 0 = 256                            */
     uint32_t keep_f64;        /* also keep FP64 coefficients              */
@@ -332,9 +337,11 @@ vlb_status  vlb_batch_use_graph(vlb_batch* b, int32_t on);
 /* Coefficients of job `job`: [P][K][3] in the job's probe order. */
 vlb_status  vlb_batch_download(vlb_batch* b, uint32_t job, float* out, size_t out_len);
 vlb_status  vlb_batch_download_f64(vlb_batch* b, uint32_t job, double* out, size_t out_len);
-/* Autotuner: times the lane maps x refill thresholds (trace_variant 13..24)
- * x work-fetch chunks {256, 1024} on a sample of every job's tiles (about
- * 16M primary rays), `iters` runs each, and keeps the fastest. */
+/* Autotuner: times the wavefront lane maps x refill thresholds
+ * (trace_variant 13..24) x work-fetch chunks {256, 1024} and the fused
+ * kernel's lane maps 2, 3 at 6 and 8 waves/SIMD (7, 8, 11, 12) on a sample of
+ * every job's tiles (about 16M primary rays), `iters` runs each, and keeps
+ * the fastest. */
 vlb_status  vlb_batch_autotune(vlb_batch* b, uint32_t iters, uint32_t* variant,
                                uint32_t* chunk);
 /* One counting pass (node visits, triangle tests, shadow rays). */
@@ -364,7 +371,7 @@ typedef struct vlb_server_options {
     uint32_t max_active;      /* requests on the device at once; 0 = 64   */
     uint32_t max_lights;      /* shadow rays per primary ray; 0 = 4       */
     int32_t  precision;       /* VLB_PREC_* of every request              */
-    uint32_t trace_variant;   /* 13..24; 0 = 20                           */
+    uint32_t trace_variant;   /* 1..24 as in vlb_batch_options; 0 = 12    */
     uint32_t chunk;           /* rays per work fetch; 0 = 256             */
     uint32_t max_depth;       /* deepest scene BVH served; 0 = 64         */
     uint32_t policy;          /* VLB_SCHED_*: how a step's rays are shared */
@@ -491,7 +498,9 @@ vlb_status  vlb_probe_lookup(vlb_ctx* ctx, uint32_t scene, const vlb_probe_grid_
  * are traced like shadow rays (and counted as such).
  * coeffs: [P][K][3] host floats to start from, or NULL to start from the
  * plan's last result (the plan must have executed).  ip NULL: back to
- * direct light only.  No counterpart call in the reference. */
+ * direct light only.  A plan created with trace_variant 0 and no autotuner
+ * pick switches to variant 20 while the indirect term is on (12 otherwise).
+ * No counterpart call in the reference. */
 typedef struct vlb_indirect_params {
     vlb_probe_grid_desc grid;
     vlb_lookup_params   lookup;

@@ -69,7 +69,9 @@ def test_batch_jobs_equal_single_bakes_bitwise(ctx, scenes):
     jobs = _jobs(scenes)
     single = [ctx.bake(sid, p) for sid, p in jobs]
     # small rounds: jobs split across rounds, rounds hold several jobs
-    for mrr, var, chunk in [(0, 20, 256), (1 << 16, 13, 1024), (1 << 14, 18, 64)]:
+    # wavefront and fused kernels
+    for mrr, var, chunk in [(0, 20, 256), (1 << 16, 13, 1024), (1 << 14, 18, 64), (0, 12, 256),
+                            (1 << 16, 1, 256), (1 << 14, 7, 256)]:
         b = ctx.batch(jobs, max_round_rays=mrr, trace_variant=var, chunk=chunk)
         info = b.info()
         if mrr:
@@ -143,7 +145,8 @@ def test_batch_autotune_and_counts(ctx, scenes):
     b.execute()
     before = [b.download(j) for j in range(len(jobs))]
     v, c = b.autotune(1)
-    assert 13 <= v <= 24 and c in (256, 1024)
+    # fused kernel (1..12) at chunk 256 or wavefront kernels (13..24)
+    assert (1 <= v <= 12 and c == 256) or (13 <= v <= 24 and c in (256, 1024))
     b.execute()
     for j in range(len(jobs)):
         assert np.array_equal(_bits(b.download(j)), _bits(before[j]))
@@ -182,6 +185,25 @@ def test_batch_rejects_mixed_precision_and_bad_options(ctx, scenes):
     with pytest.raises(vlb.VlbError):
         ctx.batch([a, b])
     with pytest.raises(vlb.VlbError):
-        ctx.batch([a], trace_variant=4)
+        ctx.batch([a], trace_variant=25)
     with pytest.raises(vlb.VlbError):
         ctx.batch([a], chunk=100)
+
+
+def test_context_outlives_its_batches_plans_and_servers(scenes):
+    """vlb_ctx_destroy refuses while a batch, plan or server made from the
+    context is alive; Context.close() destroys them first."""
+    c2 = vlb.Context(0)
+    (room, r0) = scenes[0]
+    sid = c2.upload_scene_dict(room)
+    p = _params(_probes(3, 8), rays=256)
+    b = c2.batch([(sid, p)])
+    pl = c2.plan(sid, p)
+    srv = vlb.Server(c2, state_budget=1 << 22, step_rays=1 << 15)
+    for obj in (b, pl, srv):
+        assert vlb.lib.vlb_ctx_destroy(c2.h) == vlb.ERR_INVALID
+        assert "destroy them first" in c2.last_error()
+        obj.destroy()
+    pl2 = c2.plan(sid, p)
+    c2.close()                      # destroys pl2, then the context
+    assert not pl2.h and not c2.h

@@ -84,6 +84,44 @@ def test_indirect_trace_variants_bitwise(ctx, room):
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), v
 
 
+def test_default_variant_follows_indirect_term(ctx, room):
+    """A plan left at trace_variant 0 (and not autotuned) runs the fused
+    kernel (12) for direct light and the wavefront kernels (20) while an
+    indirect term is on; either way the coefficients are the explicit
+    variants' bit for bit."""
+    s, sid, o = room
+    probes, g = _grid()
+    src = o.bake(_params(probes, 640), mode64=True)["coeffs"].astype(np.float32)
+    lp = vlb.lookup_params(vlb.LOOKUP_IRRADIANCE, True, 2e-3)
+
+    def grid_of(v):
+        pl = ctx.plan(sid, _params(probes, 640, trace_variant=v))
+        try:
+            return pl.stats()["grid"]
+        finally:
+            pl.destroy()
+
+    g12, g20 = grid_of(12), grid_of(20)
+    assert g12 != g20
+    pl = ctx.plan(sid, _params(probes, 640))
+    assert pl.stats()["grid"] == g12
+    pl.execute()
+    direct = pl.download()
+    pl.set_indirect(g, lp, 1.0, coeffs=src)
+    assert pl.stats()["grid"] == g20
+    pl.execute()
+    ind = pl.download()
+    pl.set_indirect(None)
+    assert pl.stats()["grid"] == g12
+    pl.destroy()
+    want, _ = _indirect_once(ctx, sid, _params(probes, 640, trace_variant=20), g, lp, src)
+    assert np.array_equal(ind.view(np.uint32), want.view(np.uint32))
+    pl = ctx.plan(sid, _params(probes, 640, trace_variant=12))
+    pl.execute()
+    assert np.array_equal(direct.view(np.uint32), pl.download().view(np.uint32))
+    pl.destroy()
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_bounce_chain(ctx, room, graph):
     """Each execute after set_indirect(coeffs=None) reads the previous result."""

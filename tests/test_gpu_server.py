@@ -63,9 +63,12 @@ def _bits(a):
     return np.ascontiguousarray(a).view(np.uint32)
 
 
-def test_server_request_equals_single_bake(ctx, scenes):
+# 0: the default wavefront kernels; 12 and 3: the fused kernel (8 and 6
+# waves/SIMD)
+@pytest.mark.parametrize("variant", [0, 12, 3])
+def test_server_request_equals_single_bake(ctx, scenes, variant):
     reqs = _requests(scenes, n=3)
-    srv = vlb.Server(ctx, step_rays=1 << 16)
+    srv = vlb.Server(ctx, step_rays=1 << 16, trace_variant=variant)
     for sid, p in reqs:
         got, info = srv.wait(srv.submit(sid, p))
         assert info["status"] == 0 and info["done_ms"] >= info["start_ms"] >= info["admit_ms"]
@@ -73,11 +76,12 @@ def test_server_request_equals_single_bake(ctx, scenes):
     srv.destroy()
 
 
-@pytest.mark.parametrize("policy", [vlb.SCHED_SRPT, vlb.SCHED_SHARE, vlb.SCHED_FIFO])
-def test_server_alone_equals_in_mix(ctx, scenes, policy):
+@pytest.mark.parametrize("policy,variant", [(vlb.SCHED_SRPT, 0), (vlb.SCHED_SHARE, 0),
+                                            (vlb.SCHED_FIFO, 0), (vlb.SCHED_SRPT, 12)])
+def test_server_alone_equals_in_mix(ctx, scenes, policy, variant):
     reqs = _requests(scenes, n=12, seed=5)
     target = 4
-    srv = vlb.Server(ctx, step_rays=1 << 15, policy=policy)
+    srv = vlb.Server(ctx, step_rays=1 << 15, policy=policy, trace_variant=variant)
     alone, _ = srv.wait(srv.submit(*reqs[target]))
     ids = [srv.submit(sid, p) for sid, p in reqs]      # all in flight together
     res = [srv.wait(i) for i in ids]

@@ -80,7 +80,11 @@ namespace {
 
 // trace_variant: 1..12 fused k_trace (1 + map + 4 * occupancy index),
 // 13..24 wavefront k_rays/k_shade (13 + map + 4 * refill index)
-constexpr uint32_t kDefaultVariant = 20;  // wavefront, lane map 3, refill at 32 idle lanes
+constexpr uint32_t kDefaultVariant = 12;  // fused, lane map 3, 8 waves/SIMD
+// with an indirect term: the eight corner visibility rays per hit serialise
+// in a fused lane, so the wavefront kernels (lane map 3, refill at 32 idle
+// lanes) are faster there
+constexpr uint32_t kIndirectDefaultVariant = 20;
 constexpr uint32_t kMaxVariant = 24;
 constexpr int kOccTab[3] = {0, 6, 8};
 constexpr int kRefillTab[3] = {16, 32, 8};
@@ -172,7 +176,8 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     uint32_t tv = p.trace_variant;
     if (tv == 0) {
         auto it = ctx.tuned.find({sc.id, R});
-        tv = it != ctx.tuned.end() ? it->second : kDefaultVariant;
+        pl.default_variant = it == ctx.tuned.end();
+        tv = pl.default_variant ? kDefaultVariant : it->second;
     }
     VLB_REQUIRE(tv >= 1 && tv <= kMaxVariant, VLB_ERR_INVALID, "trace_variant out of range");
     pl.grid = trace_grid_size(ctx, 0);
@@ -415,6 +420,14 @@ vlb_status vlb_ctx_destroy(vlb_ctx* h)
 {
     if (!h) return VLB_ERR_INVALID;
     Context* c = reinterpret_cast<Context*>(h);
+    {
+        std::lock_guard<std::recursive_mutex> lk(c->mtx);
+        if (c->children > 0) {
+            c->last_error = "context still has " + std::to_string(c->children) +
+                            " live plans, batches or servers: destroy them first";
+            return VLB_ERR_INVALID;
+        }
+    }
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->scenes.clear();
@@ -602,6 +615,7 @@ vlb_status vlb_plan_create(vlb_ctx* h, uint32_t scene, const vlb_bake_params* p,
         auto pl = std::make_unique<Plan>();
         build_plan(*c, s, *p, *pl);
         *out = reinterpret_cast<vlb_plan*>(pl.release());
+        ++c->children;
     });
 }
 
@@ -691,6 +705,7 @@ vlb_status vlb_plan_set_indirect(vlb_plan* h, const vlb_indirect_params* ip, con
             memset(&a.sp.ind, 0, sizeof a.sp.ind);
             if (pl->params.shade_mode == VLB_SHADE_LAMBERT) a.nvis = nvis0;
             pl->indirect = false;
+            if (pl->default_variant) apply_variant(ctx, *pl, kDefaultVariant);
             drop_graph(*pl);
             return;
         }
@@ -708,6 +723,8 @@ vlb_status vlb_plan_set_indirect(vlb_plan* h, const vlb_indirect_params* ip, con
         if (coeffs) VLB_HIP(hipMemcpyAsync(pl->ind.p, coeffs, n * sizeof(float), hipMemcpyHostToDevice, ctx.stream));
         else launch_copy_f32(pl->ind.as<float>(), pl->out.as<float>(), n, ctx.stream);
         const int nvis = nvis0 + (ip->lookup.visibility ? 8 : 0);
+        a.nvis = nvis;
+        if (pl->default_variant) apply_variant(ctx, *pl, kIndirectDefaultVariant);
         if (pl->hits.p && pl->vis.bytes < (size_t)a.ntrace * 512 * (size_t)nvis) {
             pl->vis = dev_alloc(ctx, (size_t)a.ntrace * 512 * (size_t)nvis);
             a.vis = pl->vis.as<uint8_t>();
@@ -773,7 +790,10 @@ vlb_status vlb_plan_autotune(vlb_plan* h, uint32_t iters, uint32_t* chosen)
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         apply_variant(c, *pl, best_v);
-        c.tuned[{pl->scene->id, (uint32_t)pl->args.R}] = best_v;
+        pl->default_variant = false;
+        // a pick made with the indirect term on does not carry over to
+        // direct-only plans of the scene
+        if (!pl->indirect) c.tuned[{pl->scene->id, (uint32_t)pl->args.R}] = best_v;
         if (chosen) *chosen = best_v;
     });
 }
@@ -793,6 +813,7 @@ vlb_status vlb_plan_destroy(vlb_plan* h)
     std::lock_guard<std::recursive_mutex> lk(c->mtx);
     (void)hipStreamSynchronize(c->stream);
     delete pl;
+    --c->children;
     return VLB_OK;
 }
 

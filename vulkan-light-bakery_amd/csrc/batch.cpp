@@ -187,6 +187,10 @@ void RoundBufs::alloc(Context& ctx, size_t cap, int nv, uint32_t dep)
     nvis = nv;
     depth = dep;
     grid = multi_grid_size(ctx);
+    fgrid[0] = multi_fused_grid_size(ctx, 0);
+    fgrid[1] = multi_fused_grid_size(ctx, 6);
+    fgrid[2] = multi_fused_grid_size(ctx, 8);
+    const int max_grid = std::max(std::max(grid, fgrid[0]), std::max(fgrid[1], fgrid[2]));
     shade_grid = (int)std::max<size_t>(8, std::min<size_t>(cap * 2, 8192));
     pgrid = ctx.num_cus * 8;
     mgrid = 1024;
@@ -197,11 +201,11 @@ void RoundBufs::alloc(Context& ctx, size_t cap, int nv, uint32_t dep)
     counters = dev_alloc(ctx, 4 * sizeof(unsigned long long));
     const uint32_t sb = stack_bound(dep);
     const uint32_t e = sb > (uint32_t)kLdsDepth ? sb - kLdsDepth : 0;
-    spill = dev_alloc(ctx, std::max<size_t>(16, (size_t)e * grid * 256 * 4));
+    spill = dev_alloc(ctx, std::max<size_t>(16, (size_t)e * max_grid * 256 * 4));
 }
 
 MultiArgs RoundBufs::args(const JobDesc* jobs, const uint8_t* table, const size_t off[3], int map,
-                          int refill, int chunk) const
+                          int refill, int chunk, int focc) const
 {
     MultiArgs a;
     memset(&a, 0, sizeof a);
@@ -218,23 +222,33 @@ MultiArgs RoundBufs::args(const JobDesc* jobs, const uint8_t* table, const size_
     a.chunk = chunk;
     a.wctr = wctr.as<unsigned long long>();
     a.spill = spill.as<uint32_t>();
-    a.spill_stride = (uint32_t)grid * 256u;
+    a.focc = focc;
+    a.fgrid = focc < 0 ? 0 : fgrid[focc == 8 ? 2 : (focc == 6 ? 1 : 0)];
+    a.spill_stride = (uint32_t)(focc < 0 ? grid : a.fgrid) * 256u;
     a.counters = counters.as<unsigned long long>();
     return a;
 }
 
-void multi_variant(uint32_t v, int& map, int& refill)
+void multi_variant(uint32_t v, int& map, int& refill, int& focc)
 {
-    VLB_REQUIRE(v >= 13 && v <= 24, VLB_ERR_INVALID, "multi-job trace_variant must be 13..24 (wavefront)");
+    VLB_REQUIRE(v >= 1 && v <= 24, VLB_ERR_INVALID, "multi-job trace_variant must be 1..24");
     static const int kRefill[3] = {16, 32, 8};
-    map = (int)((v - 13) % 4);
-    refill = kRefill[(v - 13) / 4];
+    static const int kOcc[3] = {0, 6, 8};
+    if (v >= 13) {
+        map = (int)((v - 13) % 4);
+        refill = kRefill[(v - 13) / 4];
+        focc = -1;
+    } else {
+        map = (int)((v - 1) % 4);
+        refill = 32;
+        focc = kOcc[(v - 1) / 4];
+    }
 }
 
 // ------------------------------------------------------------------ batch
 namespace {
 
-constexpr uint32_t kBatchDefaultVariant = 20;
+constexpr uint32_t kBatchDefaultVariant = 12;  // fused, lane map 3, 8 waves/SIMD
 constexpr uint64_t kDefaultRoundRays = 1ull << 28;
 
 struct Batch {
@@ -250,7 +264,7 @@ struct Batch {
     DevBuf table;
     RoundBufs rb;
     uint32_t variant = kBatchDefaultVariant;
-    int map = 3, refill = 32, chunk = 256;
+    int map = 3, refill = 32, chunk = 256, focc = -1;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -277,7 +291,7 @@ struct Batch {
     {
         const size_t off[3] = {round_off[r] + parts[r][0], round_off[r] + parts[r][1],
                                round_off[r] + parts[r][2]};
-        return rb.args(jobtab.as<JobDesc>(), table.as<uint8_t>(), off, map, refill, chunk);
+        return rb.args(jobtab.as<JobDesc>(), table.as<uint8_t>(), off, map, refill, chunk, focc);
     }
     void launch_all(bool count, hipStream_t st)
     {
@@ -340,7 +354,7 @@ void build_batch(Context& ctx, const vlb_batch_job* jobs, uint32_t njobs, const 
     uint64_t max_rays = opt && opt->max_round_rays ? opt->max_round_rays : kDefaultRoundRays;
     max_rays = std::min<uint64_t>(max_rays, 1ull << 30);
     b.variant = opt && opt->trace_variant ? opt->trace_variant : kBatchDefaultVariant;
-    multi_variant(b.variant, b.map, b.refill);
+    multi_variant(b.variant, b.map, b.refill, b.focc);
     b.chunk = opt && opt->chunk ? (int)opt->chunk : 256;
     VLB_REQUIRE(b.chunk >= 64 && b.chunk <= 1024 && 1024 % b.chunk == 0, VLB_ERR_INVALID,
                 "chunk must divide 1024 and be at least 64");
@@ -455,6 +469,7 @@ vlb_status vlb_batch_create(vlb_ctx* h, const vlb_batch_job* jobs, uint32_t njob
         return s;
     }
     *out = reinterpret_cast<vlb_batch*>(b);
+    ++c->children;
     return VLB_OK;
 }
 
@@ -538,27 +553,35 @@ vlb_status vlb_batch_autotune(vlb_batch* h, uint32_t iters, uint32_t* variant, u
         double best = 1e30;
         uint32_t best_v = b->variant;
         int best_c = b->chunk;
+        // wavefront: every lane map x refill x work chunk {256, 1024}; fused
+        // (one chunk size: it fetches 256 rays per workgroup): lane maps 2, 3
+        // at 6 and 8 waves/SIMD
+        std::vector<std::pair<uint32_t, int>> cand;
+        for (uint32_t v = 13; v <= 24; ++v)
+            for (int ch : {256, 1024}) cand.push_back({v, ch});
+        for (uint32_t v : {7u, 8u, 11u, 12u}) cand.push_back({v, 256});
         try {
-            for (uint32_t v = 13; v <= 24; ++v)
-                for (int ch : {256, 1024}) {
-                    int map, refill;
-                    multi_variant(v, map, refill);
-                    const MultiArgs a = b->rb.args(b->jobtab.as<JobDesc>(), tab.as<uint8_t>(), off, map,
-                                                   refill, ch);
-                    launch_multi_trace(a, false, b->rb.grid, b->rb.shade_grid, st);  // warm-up
-                    VLB_HIP(hipEventRecord(e0, st));
-                    for (uint32_t i = 0; i < iters; ++i)
-                        launch_multi_trace(a, false, b->rb.grid, b->rb.shade_grid, st);
-                    VLB_HIP(hipEventRecord(e1, st));
-                    VLB_HIP(hipEventSynchronize(e1));
-                    float ms = 0.f;
-                    VLB_HIP(hipEventElapsedTime(&ms, e0, e1));
-                    if (ms < best) {
-                        best = ms;
-                        best_v = v;
-                        best_c = ch;
-                    }
+            for (const auto& vc : cand) {
+                const uint32_t v = vc.first;
+                const int ch = vc.second;
+                int map, refill, focc;
+                multi_variant(v, map, refill, focc);
+                const MultiArgs a = b->rb.args(b->jobtab.as<JobDesc>(), tab.as<uint8_t>(), off, map,
+                                               refill, ch, focc);
+                launch_multi_trace(a, false, b->rb.grid, b->rb.shade_grid, st);  // warm-up
+                VLB_HIP(hipEventRecord(e0, st));
+                for (uint32_t i = 0; i < iters; ++i)
+                    launch_multi_trace(a, false, b->rb.grid, b->rb.shade_grid, st);
+                VLB_HIP(hipEventRecord(e1, st));
+                VLB_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                VLB_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    best_v = v;
+                    best_c = ch;
                 }
+            }
         } catch (...) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
@@ -568,7 +591,7 @@ vlb_status vlb_batch_autotune(vlb_batch* h, uint32_t iters, uint32_t* variant, u
         (void)hipEventDestroy(e1);
         VLB_HIP(hipStreamSynchronize(st));
         b->variant = best_v;
-        multi_variant(best_v, b->map, b->refill);
+        multi_variant(best_v, b->map, b->refill, b->focc);
         b->chunk = best_c;
         b->drop_graph();
         if (variant) *variant = best_v;
@@ -624,6 +647,7 @@ vlb_status vlb_batch_destroy(vlb_batch* h)
     std::lock_guard<std::recursive_mutex> lk(c->mtx);
     (void)hipStreamSynchronize(c->stream);
     delete b;
+    --c->children;
     return VLB_OK;
 }
 

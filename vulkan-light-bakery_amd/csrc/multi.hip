@@ -8,7 +8,8 @@
 //   k_rays_jobs<true>   shadow query of every (light, ray) pair
 //   k_shade_jobs        radiance per ray (MFMA operand order)
 //   k_project_jobs      SH projection of every projection tile (MFMA)
-// and k_merge_jobs merges the FP32 (sum, compensation) partials of finished
+// (k_fused_jobs replaces the first three for trace_variant 1..12) and
+// k_merge_jobs merges the FP32 (sum, compensation) partials of finished
 // jobs in FP64.  The counts of a round live in device memory (WorkHeader),
 // so one captured hipGraph replays rounds of any size.
 //
@@ -246,6 +247,83 @@ __global__ __launch_bounds__(256) void k_rays_jobs(MultiArgs a, const JobDesc* _
     }
 }
 
+// Fused form of the three passes above (trace_variant 1..12): each lane takes
+// one primary ray and runs closest hit, shading and its shadow rays with the
+// single-bake kernel's shade_ray, writing radiance in MFMA operand order.  A
+// workgroup fetches 256 items at a time from its XCD's counter and then from
+// the other XCDs' (a chunk lies within a pair of trace tiles, hence one
+// segment and one job).  Same values as the wavefront passes, bit for bit.
+template <bool COUNT, int OCC>
+__global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_fused_jobs(MultiArgs a,
+                                                                       const JobDesc* __restrict__ jobs,
+                                                                       const Segment* __restrict__ seg,
+                                                                       const WorkHeader* __restrict__ hdr)
+{
+    __shared__ uint32_t lds[kLdsDepth * 256];
+    __shared__ unsigned long long fetched;
+    const int tid = threadIdx.x;
+    Stack st;
+    st.lds = &lds[tid];
+    st.spill = a.spill + (size_t)blockIdx.x * 256 + tid;
+    st.spill_stride = a.spill_stride;
+    TraceCount cnt;
+    uint32_t nshadow = 0;
+    const int nseg = hdr->nseg;
+    const size_t nitems = (size_t)hdr->ntrace * 512;
+    // eight 256-aligned ranges, one per XCD (nitems is a multiple of 512)
+    const size_t per_xcd = ((nitems + 7) / 8 + 255) / 256 * 256;
+    const int xcd = blockIdx.x & 7;
+    int victim = 0;  // thread 0 only: ranges tried, own first
+    for (;;) {
+        if (tid == 0) {
+            unsigned long long b = ~0ull;
+            while (victim < 8) {
+                const int v = (xcd + victim) & 7;
+                const size_t vbeg = (size_t)v * per_xcd;
+                const size_t vend = min(nitems, vbeg + per_xcd);
+                if (vbeg < vend) {
+                    const unsigned long long base = atomicAdd(&a.wctr[v], 256ull);
+                    if (vbeg + base < vend) {
+                        b = vbeg + base;
+                        break;
+                    }
+                }
+                ++victim;
+            }
+            fetched = b;
+        }
+        __syncthreads();
+        const unsigned long long b = fetched;
+        __syncthreads();
+        if (b == ~0ull) break;
+        const size_t it = (size_t)b + tid;
+        int tile, l, e;
+        tile_map(a.map, it, tile, l, e);
+        const int tile0 = __builtin_amdgcn_readfirstlane((int)(b >> 10) * 2);
+        const int s = seg_of_tile(seg, nseg, tile0);
+        const JobDesc& J = jobs[__builtin_amdgcn_readfirstlane(seg[s].job)];
+        int probe, r;
+        decode_slot(J, seg[s].tb, seg[s].pt0, tile, l, e, probe, r);
+        const float4 dd = J.dirs[r];
+        float3 L = make_float3(0.f, 0.f, 0.f);
+        if (probe < J.P && dd.w != 0.0f) {
+            const float4 pp = J.probes[probe];
+            int hp = -1;
+            L = shade_ray<COUNT>(J.sc, J.sp, make_float3(pp.x, pp.y, pp.z), make_float3(dd.x, dd.y, dd.z),
+                                 J.tmin, J.tmax, st, cnt, nshadow, hp);
+        }
+        float* out = a.rad + (size_t)tile * 1536 + l * 8 + e;
+        out[0] = L.x;
+        out[512] = L.y;
+        out[1024] = L.z;
+    }
+    if (COUNT) {
+        atomicAdd(&a.counters[0], (unsigned long long)cnt.nodes);
+        atomicAdd(&a.counters[1], (unsigned long long)cnt.tris);
+        atomicAdd(&a.counters[2], (unsigned long long)nshadow);
+    }
+}
+
 // Radiance per ray of the round, in MFMA operand order.  A workgroup's 256
 // slots lie in one trace tile, hence one segment and one job.
 __global__ __launch_bounds__(256) void k_shade_jobs(MultiArgs a, const JobDesc* __restrict__ jobs,
@@ -353,6 +431,15 @@ int occupancy_of(K kernel, int cus)
 
 }  // namespace
 
+int multi_fused_grid_size(Context& ctx, int occ)
+{
+    int g;
+    if (occ == 8) g = occupancy_of(k_fused_jobs<false, 8>, ctx.num_cus);
+    else if (occ == 6) g = occupancy_of(k_fused_jobs<false, 6>, ctx.num_cus);
+    else g = occupancy_of(k_fused_jobs<false, 0>, ctx.num_cus);
+    return (g + 7) & ~7;
+}
+
 int multi_grid_size(Context& ctx)
 {
     const int g = std::max(occupancy_of(k_rays_jobs<false, false>, ctx.num_cus),
@@ -366,6 +453,15 @@ void launch_multi_trace(const MultiArgs& a, bool count, int grid, int shade_grid
     VLB_REQUIRE(a.chunk >= 64 && a.chunk <= 1024 && (1024 % a.chunk) == 0, VLB_ERR_INVALID,
                 "work chunk must divide 1024");
     k_zero_ctr<<<1, 64, 0, st>>>(a.wctr, 16);
+    if (a.focc >= 0) {
+        VLB_REQUIRE(a.fgrid % 8 == 0 && a.fgrid > 0, VLB_ERR_INVALID, "fused grid must be a multiple of 8");
+        if (count) k_fused_jobs<true, 0><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
+        else if (a.focc == 8) k_fused_jobs<false, 8><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
+        else if (a.focc == 6) k_fused_jobs<false, 6><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
+        else k_fused_jobs<false, 0><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
+        VLB_HIP(hipGetLastError());
+        return;
+    }
     if (count) k_rays_jobs<false, true><<<grid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
     else k_rays_jobs<false, false><<<grid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
     VLB_HIP(hipGetLastError());

@@ -80,14 +80,20 @@ struct RoundBufs {
     int nvis = 0;             // visibility bytes per ray
     uint32_t depth = 0;       // deepest BVH served
     int grid = 0, shade_grid = 0, pgrid = 0, mgrid = 0;
+    int fgrid[3] = {0, 0, 0};   // fused kernel grids at occupancy 0, 6, 8
     DevBuf rad, hits, vis, wctr, spill, counters;
     void alloc(Context& ctx, size_t cap_tiles, int nvis, uint32_t depth);
     // kernel arguments for the given device tables
+    // kernel arguments for the given device tables; focc: -1 for the
+    // wavefront kernels, else the fused kernel's occupancy (0, 6, 8)
     MultiArgs args(const JobDesc* jobs, const uint8_t* table, const size_t off[3], int map,
-                   int refill, int chunk) const;
+                   int refill, int chunk, int focc) const;
 };
 
-// trace_variant 13..24 -> lane map and refill threshold of the wavefront kernels.
-void multi_variant(uint32_t v, int& map, int& refill);
+// trace_variant -> lane map, refill threshold and fused occupancy: 1..12 the
+// fused kernel (1 + map + 4 * occupancy index, occupancy {0, 6, 8}; focc =
+// that occupancy), 13..24 the wavefront kernels (13 + map + 4 * refill
+// index, refill {16, 32, 8}; focc = -1).
+void multi_variant(uint32_t v, int& map, int& refill, int& focc);
 
 }  // namespace vlb

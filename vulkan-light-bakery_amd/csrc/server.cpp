@@ -149,7 +149,7 @@ struct Server {
     std::vector<int> free_slots;
     RayCache cache;
     RoundBufs rb;
-    int map = 3, refill = 32, chunk = 256;
+    int map = 3, refill = 32, chunk = 256, focc = -1;
     size_t cap_tiles = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -314,7 +314,8 @@ void Server::launch_step(const RoundTable& rt, const std::vector<Upload>& ups, u
     for (const Upload& u : ups)
         VLB_HIP(hipMemcpyAsync(u.dst, stg + u.src, u.bytes, hipMemcpyHostToDevice, st));
     if (!gexec) {
-        const MultiArgs a = rb.args(dtab.as<JobDesc>(), dtab.as<uint8_t>() + jobtab_bytes, off, map, refill, chunk);
+        const MultiArgs a = rb.args(dtab.as<JobDesc>(), dtab.as<uint8_t>() + jobtab_bytes, off, map, refill, chunk,
+                                    focc);
         VLB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         try {
             launch_multi_trace(a, false, rb.grid, rb.shade_grid, st);
@@ -528,7 +529,7 @@ vlb_status vlb_server_create(vlb_ctx* h, const vlb_server_options* o, vlb_server
         if (!op.step_rays) op.step_rays = 1ull << 25;
         if (!op.max_active) op.max_active = 64;
         if (!op.max_lights) op.max_lights = 4;
-        if (!op.trace_variant) op.trace_variant = 20;
+        if (!op.trace_variant) op.trace_variant = 12;  // fused, lane map 3, 8 waves/SIMD
         if (!op.chunk) op.chunk = 256;
         if (!op.max_depth) op.max_depth = 64;
         if (!op.result_bytes) op.result_bytes = 256ull << 20;
@@ -539,7 +540,7 @@ vlb_status vlb_server_create(vlb_ctx* h, const vlb_server_options* o, vlb_server
                     "bad precision");
         VLB_REQUIRE(op.step_rays <= (1ull << 30), VLB_ERR_INVALID, "step_rays must be <= 2^30");
         VLB_REQUIRE(op.policy <= VLB_SCHED_FIFO, VLB_ERR_INVALID, "bad scheduling policy");
-        multi_variant(op.trace_variant, s->map, s->refill);
+        multi_variant(op.trace_variant, s->map, s->refill, s->focc);
         s->chunk = (int)op.chunk;
         VLB_REQUIRE(s->chunk >= 64 && s->chunk <= 1024 && 1024 % s->chunk == 0, VLB_ERR_INVALID,
                     "chunk must divide 1024 and be at least 64");
@@ -571,6 +572,7 @@ vlb_status vlb_server_create(vlb_ctx* h, const vlb_server_options* o, vlb_server
         return st;
     }
     *out = reinterpret_cast<vlb_server*>(s);
+    ++c->children;
     return VLB_OK;
 }
 
@@ -684,7 +686,10 @@ vlb_status vlb_server_stats_get(vlb_server* h, vlb_server_stats* out)
 vlb_status vlb_server_destroy(vlb_server* h)
 {
     if (!h) return VLB_ERR_INVALID;
+    Context* c = as_server(h)->ctx;
     destroy_server(as_server(h));
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    --c->children;
     return VLB_OK;
 }
 

@@ -127,6 +127,9 @@ struct Context {
     std::map<void*, DevBuf> user_bufs;
     // autotuner decisions: (scene id, rays per probe) -> trace_variant
     std::map<std::pair<uint32_t, uint32_t>, uint32_t> tuned;
+    // live plans, batches and servers: they use the stream and the memory
+    // accounting, so the context outlives them (guarded by mtx)
+    int children = 0;
     std::recursive_mutex mtx;
 };
 
@@ -269,6 +272,9 @@ struct Plan {
     // result, refreshed at the end of every execute)
     DevBuf ind;
     bool indirect = false;
+    // trace_variant left to the library (0 at creation, no autotuner pick):
+    // the default follows the indirect term on and off
+    bool default_variant = false;
     ~Plan();
 };
 
@@ -331,6 +337,9 @@ struct MultiArgs {
     uint32_t* spill;
     uint32_t spill_stride;
     unsigned long long* counters;  // node visits, tri tests, shadow rays (count mode)
+    int focc;                 // -1: wavefront kernels; else the fused kernel at this
+                              // occupancy (0 = compiler's choice, 6, 8 waves/SIMD)
+    int fgrid;                // grid of the fused kernel
 };
 
 // ------------------------------------------------------ probe-grid lookup
@@ -372,6 +381,7 @@ void launch_trace_occluded(const SceneView& sc, const float* org, const float* d
                            uint32_t* spill, uint32_t spill_stride, int grid, hipStream_t st);
 // multi.hip
 int  multi_grid_size(Context& ctx);
+int  multi_fused_grid_size(Context& ctx, int occ);
 void launch_multi_trace(const MultiArgs& a, bool count, int grid, int shade_grid, hipStream_t st);
 void launch_multi_project(const MultiArgs& a, int precision, int grid, hipStream_t st);
 void launch_multi_merge(const MultiArgs& a, int grid, hipStream_t st);

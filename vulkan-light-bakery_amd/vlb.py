@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -608,6 +609,7 @@ class Server:
         if st:
             raise VlbError(st, f"server_create: {ctx.last_error()}")
         self._shape = {}
+        ctx._adopt(self)
 
     def submit(self, sid: int, params: Params) -> int:
         p = params.to_c()
@@ -658,10 +660,19 @@ class Context:
         if st:
             raise VlbError(st, "vlb_ctx_create")
         self.h = h
+        # plans, batches and servers still alive: vlb_ctx_destroy refuses
+        # while any is, so close() destroys them first
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, child):
+        self._children.add(child)
+        return child
 
     def close(self):
         if getattr(self, "h", None):
-            lib.vlb_ctx_destroy(self.h)
+            for ch in list(self._children):
+                ch.destroy()
+            self._chk(lib.vlb_ctx_destroy(self.h), "ctx_destroy")
             self.h = None
 
     def __enter__(self):
@@ -756,7 +767,7 @@ class Context:
         p = params.to_c()
         h = C.c_void_p()
         self._chk(lib.vlb_plan_create(self.h, sid, C.byref(p), C.byref(h)), "plan_create")
-        return Plan(self, h, p.num_probes, params.K)
+        return self._adopt(Plan(self, h, p.num_probes, params.K))
 
     def bake_bounces(self, sid: int, params: Params, grid: "ProbeGridDesc", bounces: int,
                      lp: "LookupParams" = None, scale=1.0) -> list:
@@ -792,7 +803,7 @@ class Context:
         self._chk(lib.vlb_batch_create(self.h, arr, len(jobs), C.byref(opt), C.byref(h)),
                   "batch_create")
         shapes = [(cp.num_probes, prm.K, 3) for (_, prm), cp in zip(jobs, cps)]
-        return Batch(self, h, shapes)
+        return self._adopt(Batch(self, h, shapes))
 
     def bake_radiance(self, sid: int, params: Params, want_radiance=True, want_hits=True):
         p = params.to_c()
