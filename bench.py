@@ -30,8 +30,14 @@ sys.path.insert(0, os.path.join(ROOT, "vulkan-light-bakery_amd"))
 
 METRIC = "probes/sec & Grays/s at ≤1e-3 rel SH err vs FP64; p50/p99 rebake latency"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 112.0   # bytes fetched per BVH4 node visit (7 x 16 B)
 TRI_BYTES = 48.0     # bytes per triangle test (3 x float4)
+
+
+def node_bytes(variant):
+    """Bytes fetched per BVH4 node visit: the fused kernels (trace_variant
+    1..12) read float nodes (7 x 16 B in use), the wavefront kernels
+    (13..24) quantized ones (4 x 16 B)."""
+    return 112.0 if variant <= 12 else 64.0
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16_fast": 3, "fp16_fast": 4}
 SCHED = {"srpt": 0, "share": 1, "fifo": 2}
 
@@ -106,10 +112,10 @@ def bench_c2(args):
     rays_per_step = cnt["primary_rays"] + cnt["shadow_rays"]
     grays = rays_per_step * args.steps / elapsed / 1e9
 
-    # algorithmic bytes of one trace stage: 112 B per BVH4 node fetched +
-    # 48 B per triangle tested (the 48 B shading record per primary hit is
+    # algorithmic bytes of one trace stage: the node bytes of the variant
+    # per BVH4 node fetched + 48 B per triangle tested (the 48 B shading record per primary hit is
     # not counted), both counted by the kernels' counting variant
-    alg_bytes = NODE_BYTES * cnt["node_visits"] + TRI_BYTES * cnt["tri_tests"]
+    alg_bytes = node_bytes(variant) * cnt["node_visits"] + TRI_BYTES * cnt["tri_tests"]
     trace_s = st["trace_ms"] / 1e3
     achieved = alg_bytes / trace_s / 1e9
     traffic = load_traffic("bake_c2_traffic.json")
@@ -355,7 +361,7 @@ def bench_c3(args):
             band["all"] = float(rel_err(got, ref).max())
             errs[f"L{order}_{name}"] = band
     rays = cnt["primary_rays"] + cnt["shadow_rays"]
-    alg = NODE_BYTES * cnt["node_visits"] + TRI_BYTES * cnt["tri_tests"]
+    alg = node_bytes(variant) * cnt["node_visits"] + TRI_BYTES * cnt["tri_tests"]
     tr = runs["bf16"]["trace_ms"] / 1e3
     line = {"metric": METRIC, "value": runs["bf16"]["probes_per_s"], "unit": "probes/s", "n_gpus": 1,
             "steps": args.steps, "warmup": 1, "ms_per_step": runs["bf16"]["ms_per_step"],

@@ -171,7 +171,9 @@ typedef struct vlb_bake_params {
                                       /*   shadow, shade): 13 + map +        */
                                       /*   4*refill, refill {16, 32, 8}      */
                                       /*   idle lanes; 0 = autotuned choice, */
-                                      /*   else 12 (20 with an indirect term)*/
+                                      /*   else 12 below 2^24 primary rays,  */
+                                      /*   20 from there on or with an       */
+                                      /*   indirect term                     */
     uint32_t         reserved[2];
 } vlb_bake_params;
 
@@ -220,9 +222,11 @@ vlb_status  vlb_device_info(vlb_ctx* ctx, int32_t* num_cus, char* name, size_t n
 /* ------------------------------------------------------------- scene API */
 /* Replaces SceneManager::pushScene + Scene_t::buildAccelerationStructures
  * (src/scene_manager.cpp:1013-1034, 339-443): world-space triangles are
- * uploaded and a BVH is built on the GPU (LBVH, Karras 2012).
- * xyz: nverts*3, idx: ntris*3, normals: nverts*3 or NULL (geometric
- * normals), albedo_rgb: ntris*3 (base colour, env_map.rchit:36-49). */
+ * uploaded and a BVH is built on the GPU (PLOC, Meister & Bittner 2018,
+ * collapsed to a 4-wide BVH kept as float and as 8-bit quantized nodes).
+ * xyz: nverts*3, finite with |x| <= 2^100 (else VLB_ERR_INVALID),
+ * idx: ntris*3, normals: nverts*3 or NULL (geometric normals),
+ * albedo_rgb: ntris*3 (base colour, env_map.rchit:36-49). */
 vlb_status  vlb_scene_upload(vlb_ctx* ctx, const float* xyz, uint32_t nverts,
                              const uint32_t* idx, uint32_t ntris,
                              const float* normals, const float* albedo_rgb,
@@ -499,7 +503,7 @@ vlb_status  vlb_probe_lookup(vlb_ctx* ctx, uint32_t scene, const vlb_probe_grid_
  * coeffs: [P][K][3] host floats to start from, or NULL to start from the
  * plan's last result (the plan must have executed).  ip NULL: back to
  * direct light only.  A plan created with trace_variant 0 and no autotuner
- * pick switches to variant 20 while the indirect term is on (12 otherwise).
+ * pick runs variant 20 while the indirect term is on.
  * No counterpart call in the reference. */
 typedef struct vlb_indirect_params {
     vlb_probe_grid_desc grid;

@@ -309,6 +309,36 @@ def test_wavefront_radiance_and_hits(ctx, room, variant):
     np.testing.assert_allclose(rad, ref["radiance"], rtol=2e-6, atol=1e-7)
 
 
+# scale and offset of the room: tiny, far from the origin, large.  The
+# wavefront kernels read the quantized BVH4 (8-bit child boxes on per-node
+# grids), the fused ones the float BVH4; both must give the oracle's hits.
+@pytest.mark.parametrize("scale,offset", [(1e-3, (0.0, 0.0, 0.0)), (1.0, (1e4, -2e4, 3e4)),
+                                          (50.0, (0.0, 0.0, 0.0))])
+def test_quantized_and_float_bvh4_hits_at_scale(ctx, room, scale, offset):
+    s, _, _ = room
+    off = np.asarray(offset, np.float32)
+    xyz = (np.asarray(s["xyz"], np.float32) * np.float32(scale) + off).astype(np.float32)
+    sid = ctx.upload_scene(xyz, s["idx"], normals=s.get("normals"), albedo=s.get("albedo"))
+    o = orc.Scene(xyz, s["idx"], normals=s.get("normals"), albedo=s.get("albedo"))
+    probes = (_probes(24, seed=13) * np.float32(scale) + off).astype(np.float32)
+    for v in (12, 16):
+        p = _bake_params(probes, rays=300, trace_variant=v, shadow_bias=1e-3 * scale)
+        rad, hits = ctx.bake_radiance(sid, p)
+        ref = o.bake(p, mode64=False, want_radiance=True, want_hits=True)
+        assert np.array_equal(hits, ref["hits"]), v
+        assert (hits >= 0).mean() > 0.5
+        np.testing.assert_allclose(rad, ref["radiance"], rtol=2e-6, atol=1e-7)
+    ctx.release_scene(sid)
+
+
+def test_scene_upload_rejects_non_finite_or_huge_vertices(ctx):
+    idx = np.array([[0, 1, 2]], np.uint32)
+    for bad in (np.nan, np.inf, 2.0 ** 101):
+        xyz = np.array([[0, 0, 0], [1, 0, 0], [0, bad, 0]], np.float32)
+        with pytest.raises(vlb.VlbError):
+            ctx.upload_scene(xyz, idx)
+
+
 @pytest.mark.parametrize("P,R", [(1, 1), (17, 31), (33, 100)])
 def test_wavefront_ragged_and_reference(ctx, room, P, R):
     s, sid, o = room

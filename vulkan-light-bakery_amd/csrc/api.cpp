@@ -80,11 +80,20 @@ namespace {
 
 // trace_variant: 1..12 fused k_trace (1 + map + 4 * occupancy index),
 // 13..24 wavefront k_rays/k_shade (13 + map + 4 * refill index)
-constexpr uint32_t kDefaultVariant = 12;  // fused, lane map 3, 8 waves/SIMD
-// with an indirect term: the eight corner visibility rays per hit serialise
-// in a fused lane, so the wavefront kernels (lane map 3, refill at 32 idle
-// lanes) are faster there
-constexpr uint32_t kIndirectDefaultVariant = 20;
+constexpr uint32_t kFusedDefault = 12;  // fused, lane map 3, 8 waves/SIMD
+constexpr uint32_t kWaveDefault = 20;   // wavefront, lane map 3, refill at 32 idle lanes
+constexpr uint64_t kWaveFromRays = 1ull << 24;
+
+// trace_variant of a plan that leaves it to the library.  Below 2^24 primary
+// rays the fused kernel wins (the wavefront pipeline's three passes and its
+// hit-record traffic are a fixed cost); from there on the wavefront kernels'
+// refill of idle lanes wins, by up to 1.5x on 0.5-2M triangle scenes at
+// 2^28 rays.  With an indirect term the eight corner visibility rays per hit
+// serialise in a fused lane, so the wavefront kernels are used at any size.
+uint32_t default_variant(uint64_t probes, uint64_t rays, bool indirect)
+{
+    return indirect || probes * rays >= kWaveFromRays ? kWaveDefault : kFusedDefault;
+}
 constexpr uint32_t kMaxVariant = 24;
 constexpr int kOccTab[3] = {0, 6, 8};
 constexpr int kRefillTab[3] = {16, 32, 8};
@@ -177,7 +186,7 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     if (tv == 0) {
         auto it = ctx.tuned.find({sc.id, R});
         pl.default_variant = it == ctx.tuned.end();
-        tv = pl.default_variant ? kDefaultVariant : it->second;
+        tv = pl.default_variant ? default_variant(p.num_probes, R, false) : it->second;
     }
     VLB_REQUIRE(tv >= 1 && tv <= kMaxVariant, VLB_ERR_INVALID, "trace_variant out of range");
     pl.grid = trace_grid_size(ctx, 0);
@@ -472,6 +481,10 @@ vlb_status vlb_scene_upload(vlb_ctx* h, const float* xyz, uint32_t nverts, const
         VLB_REQUIRE(ntris < (1u << 30), VLB_ERR_INVALID, "too many triangles");
         for (uint64_t i = 0; i < (uint64_t)ntris * 3; ++i)
             VLB_REQUIRE(idx[i] < nverts, VLB_ERR_INVALID, "index out of range at " + std::to_string(i));
+        // the BVH4 node grids need finite boxes of bounded magnitude
+        for (uint64_t i = 0; i < (uint64_t)nverts * 3 && ntris > 0; ++i)
+            VLB_REQUIRE(std::isfinite(xyz[i]) && std::fabs(xyz[i]) <= 0x1p100f, VLB_ERR_INVALID,
+                        "vertex coordinate not finite or beyond 2^100 at " + std::to_string(i));
         VLB_HIP(hipSetDevice(c->device));
         auto s = std::make_unique<Scene>();
         build_scene_bvh(*c, *s, xyz, nverts, idx, ntris, normals, albedo);
@@ -705,7 +718,7 @@ vlb_status vlb_plan_set_indirect(vlb_plan* h, const vlb_indirect_params* ip, con
             memset(&a.sp.ind, 0, sizeof a.sp.ind);
             if (pl->params.shade_mode == VLB_SHADE_LAMBERT) a.nvis = nvis0;
             pl->indirect = false;
-            if (pl->default_variant) apply_variant(ctx, *pl, kDefaultVariant);
+            if (pl->default_variant) apply_variant(ctx, *pl, default_variant(a.P, a.R, false));
             drop_graph(*pl);
             return;
         }
@@ -724,7 +737,7 @@ vlb_status vlb_plan_set_indirect(vlb_plan* h, const vlb_indirect_params* ip, con
         else launch_copy_f32(pl->ind.as<float>(), pl->out.as<float>(), n, ctx.stream);
         const int nvis = nvis0 + (ip->lookup.visibility ? 8 : 0);
         a.nvis = nvis;
-        if (pl->default_variant) apply_variant(ctx, *pl, kIndirectDefaultVariant);
+        if (pl->default_variant) apply_variant(ctx, *pl, default_variant(a.P, a.R, true));
         if (pl->hits.p && pl->vis.bytes < (size_t)a.ntrace * 512 * (size_t)nvis) {
             pl->vis = dev_alloc(ctx, (size_t)a.ntrace * 512 * (size_t)nvis);
             a.vis = pl->vis.as<uint8_t>();

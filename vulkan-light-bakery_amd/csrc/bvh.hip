@@ -387,12 +387,58 @@ __device__ __forceinline__ void child_box(const BNode& a, int k, float b[6])
     }
 }
 
+// Grid of one axis of a BVH4 node's box [lo, hi] (finite, |x| <= 2^100):
+// origin o = lo and the smallest float step s with fma(255, s, o) >= hi.  A
+// plane q decodes as fma(q, s, o) here and in the traversal (the same
+// correctly rounded operation, non-decreasing in q), so codes chosen by
+// comparing that decoded value with the float plane are conservative
+// whatever the rounding.
+__host__ __device__ void axis_grid(float lo, float hi, float& o, float& s)
+{
+    o = lo;
+    const double want = ((double)hi - (double)lo) / 255.0;
+    s = (float)want;
+    if ((double)s < want) s = nextafterf(s, __builtin_huge_valf());
+    // s >= the exact extent / 255 makes fma(255, s, o) >= hi already; the
+    // loop only covers a rounded `want`, first in ulps, then by doubling
+    for (int i = 0; fmaf(255.0f, s, o) < hi; ++i)
+        s = i < 8 ? nextafterf(s, __builtin_huge_valf()) : fmaxf(2.0f * s, 0x1p-100f);
+}
+
+// 8-bit codes of one plane of the four children: the largest q whose plane
+// is <= the child's lo (UP false), or the smallest q whose plane is >= its
+// hi (UP true).
+template <bool UP>
+__host__ __device__ uint32_t plane_codes(const float v[4], int m, float o, float s)
+{
+    uint32_t w = 0;
+    for (int k = 0; k < m; ++k) {
+        const float x = v[k];
+        int q = 0;
+        if (s > 0.0f) {
+            const double r = ((double)x - (double)o) / (double)s;
+            q = (int)fmin(fmax(UP ? ceil(r) : floor(r), 0.0), 255.0);
+            if (UP) {
+                while (q < 255 && fmaf((float)q, s, o) < x) ++q;
+                while (q > 0 && fmaf((float)(q - 1), s, o) >= x) --q;
+            } else {
+                while (q > 0 && fmaf((float)q, s, o) > x) --q;
+                while (q < 255 && fmaf((float)(q + 1), s, o) <= x) ++q;
+            }
+        }
+        w |= (uint32_t)q << (8 * k);
+    }
+    return w;
+}
+
 // BVH2 -> BVH4: every even-depth BVH2 node becomes a BVH4 node whose
 // children are its grandchildren through inner children, or the child
-// itself when that is a leaf (2 to 4 entries).  Boxes are copied, so the
-// BVH4 is exactly as conservative as the BVH2.
+// itself when that is a leaf (2 to 4 entries).  The float node copies the
+// boxes (exactly as conservative as the BVH2); the quantized node rounds
+// them outward on the node's grid.
 __global__ void k_collapse(const BNode* __restrict__ n2, const int* __restrict__ depth,
-                           const int* __restrict__ idx4, int nin, BNode4* __restrict__ n4)
+                           const int* __restrict__ idx4, int nin, BNode4* __restrict__ n4,
+                           QNode4* __restrict__ q4)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nin || (depth[i] & 1)) return;
@@ -414,7 +460,8 @@ __global__ void k_collapse(const BNode* __restrict__ n2, const int* __restrict__
             ch[m++] = gc < 0 ? gc : idx4[gc];
         }
     }
-    for (; m < 4; ++m) {
+    const int used = m;
+    for (; m < 4; ++m) {                 // never tested (slab4)
         for (int t = 0; t < 6; ++t) bx[m][t] = 0.0f;
         ch[m] = kEmpty;
     }
@@ -428,6 +475,27 @@ __global__ void k_collapse(const BNode* __restrict__ n2, const int* __restrict__
     o.child = make_int4(ch[0], ch[1], ch[2], ch[3]);
     o.pad = make_int4(0, 0, 0, 0);
     n4[idx4[i]] = o;
+    float org[3], stp[3];
+    uint32_t code[6];
+    for (int a = 0; a < 3; ++a) {
+        float lo[4], hi[4];
+        float blo = bx[0][a], bhi = bx[0][3 + a];
+        for (int k = 0; k < used; ++k) {
+            lo[k] = bx[k][a];
+            hi[k] = bx[k][3 + a];
+            blo = fminf(blo, lo[k]);
+            bhi = fmaxf(bhi, hi[k]);
+        }
+        axis_grid(blo, bhi, org[a], stp[a]);
+        code[a] = plane_codes<false>(lo, used, org[a], stp[a]);
+        code[3 + a] = plane_codes<true>(hi, used, org[a], stp[a]);
+    }
+    QNode4 q;
+    q.a = make_float4(org[0], org[1], org[2], stp[0]);
+    q.b = make_float4(stp[1], stp[2], __uint_as_float(code[0]), __uint_as_float(code[1]));
+    q.q = make_uint4(code[2], code[3], code[4], code[5]);
+    q.child = make_int4(ch[0], ch[1], ch[2], ch[3]);
+    q4[idx4[i]] = q;
 }
 
 inline int nblk(int n, int b = 256) { return (n + b - 1) / b; }
@@ -587,8 +655,9 @@ void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
     VLB_HIP(hipStreamSynchronize(st));
     const int n4 = last[0] + last[1];
     s.nodes4 = dev_alloc(ctx, (size_t)n4 * sizeof(BNode4));
+    s.qnodes4 = dev_alloc(ctx, (size_t)n4 * sizeof(QNode4));
     k_collapse<<<nblk(nin), 256, 0, st>>>(s.nodes.as<BNode>(), idepth.as<int>(), idx4.as<int>(), nin,
-                                          s.nodes4.as<BNode4>());
+                                          s.nodes4.as<BNode4>(), s.qnodes4.as<QNode4>());
     VLB_HIP(hipGetLastError());
     VLB_HIP(hipEventRecord(e1, st));
     float hb[16];

@@ -30,17 +30,37 @@ struct __align__(16) BNode {
     int4   d;  // child0 child1 - -
 };
 
-// The traversal kernels walk a 4-wide BVH collapsed from it: one BVH4 node
-// = 128 bytes, the boxes of up to four children as structure-of-arrays
-// (lo.x of child c in lox[c], ...) and their ids: >= 0 inner BVH4 node,
-// < 0 leaf (~ leaf-order triangle), kEmpty an unused slot.  The root is
-// node 0.  Half the dependent node fetches of the BVH2 per ray.
+// The traversal kernels walk a 4-wide BVH collapsed from it (half the
+// dependent node fetches of the BVH2 per ray), stored twice with the same
+// node ids: the root is node 0; child ids >= 0 are inner BVH4 nodes, < 0
+// leaves (~ leaf-order triangle), kEmpty an unused slot.
+//
+// BNode4, 128 bytes (112 in use): the boxes of up to four children as float
+// structure-of-arrays (lo.x of child c in lox[c], ...) and their ids.  Read
+// by the fused kernels, which run at the 64-VGPR bound of 8 waves/SIMD where
+// a decode step costs more than the bytes it saves.
+//
+// QNode4, 64 bytes: per axis a grid origin o (the node box's lo) and step s
+// (its extent / 255, rounded up), the children's boxes as 8-bit grid
+// coordinates q (plane = fma(q, s, o); lo codes rounded down, hi codes
+// rounded up, each checked against that very decode when the node is
+// built), and the ids.  A decoded box contains the child's float box, so the
+// slab test sees conservative boxes, slightly larger than the BVH2's (2-6%
+// more node visits).  Read by the wavefront kernels, which have registers to
+// spare and whose node fetches are half the bytes through the texture path.
 struct __align__(16) BNode4 {
     float4 lox, loy, loz;
     float4 hix, hiy, hiz;
     int4   child;
     int4   pad;
 };
+struct __align__(16) QNode4 {
+    float4 a;      // o.x, o.y, o.z, s.x
+    float4 b;      // s.y, s.z, lo.x codes, lo.y codes (as bits)
+    uint4  q;      // lo.z, hi.x, hi.y, hi.z codes
+    int4   child;
+};
+// (codes of one plane: child k in byte k)
 constexpr int kEmpty = (int)0x80000000;
 
 // Stack entries a BVH4 traversal can hold for a BVH2 of `depth` inner
@@ -54,7 +74,8 @@ __host__ __device__ inline uint32_t stack_bound(uint32_t depth)
 // Shading record per prim id = 3 x float4: (n0, albedo.r) (n1, .g) (n2, .b).
 
 struct SceneView {
-    const BNode4* nodes;
+    const BNode4* nodes;      // float BVH4 (fused kernels)
+    const QNode4* qnodes;     // quantized BVH4, same ids (wavefront kernels)
     const float4* tris;
     const float4* shade;
     int           ntris;
@@ -263,7 +284,15 @@ __device__ __forceinline__ void load_tri(const float4* tris, int leaf, float3& v
     prim = __float_as_int(a.w);
 }
 
-// Node fetch: the 112 bytes in use (7 x 16 B).
+// Planes fma(q, s, o) of the four children for one code word (the decode
+// the node builder checked its codes against).
+__device__ __forceinline__ float4 decode4(uint32_t q, float s, float o)
+{
+    return make_float4(fmaf((float)(q & 255u), s, o), fmaf((float)((q >> 8) & 255u), s, o),
+                       fmaf((float)((q >> 16) & 255u), s, o), fmaf((float)(q >> 24), s, o));
+}
+
+// Float node fetch: the 112 bytes in use (7 x 16 B).
 __device__ __forceinline__ BNode4 load_node(const BNode4* nodes, uint32_t i)
 {
     const BNode4* p = nodes + i;
@@ -274,6 +303,24 @@ __device__ __forceinline__ BNode4 load_node(const BNode4* nodes, uint32_t i)
     n.hix = p->hix;
     n.hiy = p->hiy;
     n.hiz = p->hiz;
+    n.child = p->child;
+    return n;
+}
+
+// Quantized node fetch (4 x 16 B) and decode to the float form.
+__device__ __forceinline__ BNode4 load_node(const QNode4* nodes, uint32_t i)
+{
+    const QNode4* p = nodes + i;
+    const float4 a = p->a;
+    const float4 b = p->b;
+    const uint4 q = p->q;
+    BNode4 n;
+    n.lox = decode4(__float_as_uint(b.z), a.w, a.x);
+    n.loy = decode4(__float_as_uint(b.w), b.x, a.y);
+    n.loz = decode4(q.x, b.y, a.z);
+    n.hix = decode4(q.y, a.w, a.x);
+    n.hiy = decode4(q.z, b.x, a.y);
+    n.hiz = decode4(q.w, b.y, a.z);
     n.child = p->child;
     return n;
 }

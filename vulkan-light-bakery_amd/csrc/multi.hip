@@ -75,7 +75,7 @@ __device__ __forceinline__ void decode_slot(const JobDesc& J, int tb, int pt0, i
 // Per-lane ray of a traversal pass: its state and the BVH it walks.
 struct LaneRay {
     TravState T;
-    const BNode4* nodes;
+    const QNode4* nodes;
     const float4* tris;
     int slot, light;
 };
@@ -98,7 +98,7 @@ __device__ __forceinline__ bool setup_item(const MultiArgs& a, const JobDesc& J,
         const float4 pp = J.probes[probe];
         trav_init(R.T, J.sc.maxabs, make_float3(pp.x, pp.y, pp.z), make_float3(dd.x, dd.y, dd.z),
                   J.tmin, J.tmax);
-        R.nodes = J.sc.nodes;
+        R.nodes = J.sc.qnodes;
         R.tris = J.sc.tris;
         return true;
     }
@@ -131,7 +131,7 @@ __device__ __forceinline__ bool setup_item(const MultiArgs& a, const JobDesc& J,
         return false;
     }
     trav_init(R.T, J.sc.maxabs, s.org, s.dir, 0.0f, s.dist);
-    R.nodes = J.sc.nodes;
+    R.nodes = J.sc.qnodes;
     R.tris = J.sc.tris;
     return true;
 }

@@ -87,16 +87,17 @@ struct Scene {
     int ntris = 0;
     int nnodes = 0;           // BVH2 inner nodes (build product, exported)
     int nnodes4 = 0;          // BVH4 nodes (traversed)
-    DevBuf nodes, nodes4, tris, shade;
+    DevBuf nodes, nodes4, qnodes4, tris, shade;
     float maxabs = 0.0f;
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
     uint32_t depth = 0;
     double build_ms = 0.0;
-    size_t bytes() const { return nodes.bytes + nodes4.bytes + tris.bytes + shade.bytes; }
+    size_t bytes() const { return nodes.bytes + nodes4.bytes + qnodes4.bytes + tris.bytes + shade.bytes; }
     SceneView view() const
     {
         SceneView v;
         v.nodes = nodes4.as<BNode4>();
+        v.qnodes = qnodes4.as<QNode4>();
         v.tris = tris.as<float4>();
         v.shade = shade.as<float4>();
         v.ntris = ntris;

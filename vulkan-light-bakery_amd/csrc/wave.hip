@@ -184,7 +184,7 @@ __global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigne
         }
         // --- one traversal step for active lanes
         if (act) {
-            if (trav_step<ANY, COUNT>(T, a.sc.nodes, a.sc.tris, st, q, cnt)) {
+            if (trav_step<ANY, COUNT>(T, a.sc.qnodes, a.sc.tris, st, q, cnt)) {
                 if (ANY) {
                     a.vis[(size_t)slot * a.nvis + light] = (uint8_t)(T.best == 0 ? 1 : 0);
                 } else if (T.best >= 0) {

@@ -155,7 +155,7 @@ __device__ __forceinline__ void trav_init(TravState& T, float maxabs, const floa
 // - 4, so a write past the end stays inside the queue), hit inner children
 // go on the stack.
 template <bool COUNT>
-__device__ __forceinline__ void visit_node(TravState& T, const BNode4* nodes, const Stack& st,
+__device__ __forceinline__ void visit_node(TravState& T, const QNode4* nodes, const Stack& st,
                                            const LeafQueue& q, TraceCount& cnt)
 {
     const BNode4 n = load_node(nodes, T.node);
@@ -206,7 +206,7 @@ __device__ __forceinline__ void test_queue(TravState& T, const float4* tris, con
 // a node visit, then (wave-uniform) the queued tests when due.  Returns true
 // when the ray is finished (ANY: T.best = 0 on an accepted triangle).
 template <bool ANY, bool COUNT>
-__device__ __forceinline__ bool trav_step(TravState& T, const BNode4* nodes, const float4* tris,
+__device__ __forceinline__ bool trav_step(TravState& T, const QNode4* nodes, const float4* tris,
                                           const Stack& st, const LeafQueue& q, TraceCount& cnt)
 {
     if (T.live && T.nq <= kLeafQueue - 4) visit_node<COUNT>(T, nodes, st, q, cnt);
