@@ -236,16 +236,21 @@ def test_empty_and_single_triangle_scenes(ctx):
     ctx.release_scene(sid)
 
 
-def test_skybox_cubemap_vs_oracle(ctx):
-    rng = np.random.default_rng(1)
-    sun = vlb.sun_direction(rng)
-    cubes = np.stack([orc.procedural_cubemap(32, sun, tint=(1.0, 0.8, 0.6)),
-                      orc.procedural_cubemap(32, -sun)])
+# batches of 1 (one cubemap per workgroup), 2 and 3 (two per workgroup at
+# L2, the last one alone), power-of-two and other face sizes
+@pytest.mark.parametrize("N,batch", [(32, 2), (32, 1), (24, 3), (17, 5)])
+def test_skybox_cubemap_vs_oracle(ctx, N, batch):
+    rng = np.random.default_rng(N * batch)
+    cubes = []
+    for b in range(batch):
+        sun = vlb.sun_direction(rng)
+        cubes.append(orc.procedural_cubemap(N, sun, tint=tuple(rng.uniform(0.5, 1.0, 3))))
+    cubes = np.stack(cubes)
     for order, K in ((2, 9), (3, 16)):
         g = ctx.skybox_sh(cubes, order)
-        for b in range(2):
+        for b in range(batch):
             ref = orc.skybox_sh(cubes[b], K)
-            assert rel_err(g[b], ref).max() <= 1e-5
+            assert rel_err(g[b], ref).max() <= 1e-5, (order, b)
 
 
 @pytest.mark.parametrize("variant", [vlb.EQUIRECT_SH_COMP, vlb.EQUIRECT_SKYBOX_COMP])
