@@ -14,7 +14,7 @@ roofline figure is for the trace stage, timed with hipEvents on the
 library's own stream inside the timed region.
 
     python bench.py [--steps K] [--warmup W] [--precision fp32|bf16|fp16]
-                    [--config c2|c1|c3|c4|c5|copy|sweep|bounce] [--no-cpu-baseline]
+                    [--config c2|c1|c3|c4|c5|copy|sweep|bounce|reference] [--no-cpu-baseline]
 
 --no-cpu-baseline skips the CPU leg (the oracle's baseline timing and the
 SH error check against the FP64 oracle).
@@ -714,6 +714,84 @@ def bench_c5(args):
     return line
 
 
+def bench_reference(args):
+    """The reference's own bake settings (LightBaker's defaults,
+    light_baker.cpp:38 and 65, env_map_generator.cpp:24-28, env_map.rchit):
+    7x7x7 probes in the reference's BFS order over the scene bounds (origin
+    included, scene_manager.cpp:497-507), 3141 x 1000 equirect rays per
+    probe, the REFERENCE shading with its shadow ray and sRGB + UNORM8
+    conversion, L3 (16 coefficients, the literal constants of sh_common.h),
+    FP32, on the c2 room (the reference's glTF scenes are not in its
+    snapshot).  A step is one whole bake of 343 probes x 3.141 M rays."""
+    import numpy as np
+
+    import vlb
+
+    scene = vlb.generate_scene(vlb.SCENE_ROOM, 2)
+    lo = np.minimum(0.0, np.asarray(scene["bounds_min"], np.float64))
+    hi = np.maximum(0.0, np.asarray(scene["bounds_max"], np.float64))
+    probes, _ = vlb.probe_grid(lo.tolist(), hi.tolist(), [7, 7, 7], vlb.ORDER_REFERENCE_BFS)
+
+    def params(pr, prec):
+        return vlb.Params(probes=pr, dir_mode=vlb.DIRS_EQUIRECT, equirect_w=3141, equirect_h=1000,
+                          sh_order=3, sh_constants=vlb.SH_LITERAL, precision=prec,
+                          shade_mode=vlb.SHADE_REFERENCE, trace_variant=args.variant)
+
+    prm = params(probes, PREC[args.precision])
+    P, R = probes.shape[0], 3141 * 1000
+    ctx = vlb.Context(0)
+    sid = ctx.upload_scene_dict(scene)
+    plan = ctx.plan(sid, prm)
+    variant = plan.autotune(2) if args.variant == 0 else args.variant
+    plan.count()
+    cnt = plan.stats()
+    for _ in range(max(1, args.warmup)):
+        plan.execute()
+    plan.sync()
+    plan.set_timing(True)
+    plan.sync()
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute()
+    plan.sync()
+    el = (time.perf_counter() - t0) / steps
+    st = plan.stats()
+    got = plan.download()
+    plan.destroy()
+    ctx.close()
+    # CPU leg: two probes against the FP64 oracle, and the FP32 oracle's rate
+    orc = oracle_leg(args)
+    err, cpu = None, None
+    if orc is not None:
+        osc = orc.Scene.from_dict(scene)
+        sub = np.array([0, P // 2])
+        nth = min(16, os.cpu_count() or 1)
+        ref = osc.bake(params(probes[sub].copy(), PREC["fp32"]), mode64=True, nthreads=nth)["coeffs"]
+        e = rel_err(got[sub], ref)
+        err = {"max": float(e.max()), "probes": int(sub.size)}
+        c0 = time.perf_counter()
+        osc.bake(params(probes[sub].copy(), PREC["fp32"]), mode64=False, nthreads=nth)
+        spp = (time.perf_counter() - c0) / sub.size
+        cpu = {"value": round(1.0 / spp, 3), "unit": "probes/s", "cores": nth, "kind": "port",
+               "sample": f"FP32 C oracle, {sub.size} of the {P} probes (3.141 M rays each, per-probe cost "
+                         f"uniform in rays), {nth} threads, {spp * sub.size:.1f} s"}
+    rays = cnt["primary_rays"] + cnt["shadow_rays"]
+    return {
+        "metric": METRIC, "value": round(P / el, 1), "unit": "probes/s", "n_gpus": 1,
+        "steps": steps, "warmup": max(1, args.warmup), "ms_per_step": round(el * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else args.precision,
+        "data": "synthetic (seeded procedural room, the reference's equirect ray grid)",
+        "config": {"workload": "reference bake settings", "scene": "room seed 2", "probes": P,
+                   "rays_per_probe": R, "sh_order": 3, "shading": "reference (env_map.rchit)",
+                   "precision": args.precision, "trace_variant": variant},
+        "grays_per_s": round(rays / el / 1e9, 3), "shadow_rays_per_bake": cnt["shadow_rays"],
+        "kernel_ms": {"trace": round(st["trace_ms"], 3), "project": round(st["project_ms"], 3),
+                      "merge": round(st["merge_ms"], 3)},
+        "sh_rel_err_vs_fp64": err, "cpu_baseline": cpu}
+
+
 def bench_bounce(args):
     """Multi-bounce on the c2 room: a direct bake, then bounces over the
     plan's own 16x8x16 probe grid (irradiance lookup at every primary hit),
@@ -816,7 +894,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "copy", "sweep", "bounce"])
+    ap.add_argument("--config", default="c2",
+                    choices=["c2", "c1", "c3", "c4", "c5", "copy", "sweep", "bounce", "reference"])
     ap.add_argument("--rate", type=float, default=40.0, help="c5: request arrivals per second")
     ap.add_argument("--policy", default="srpt", choices=sorted(SCHED), help="c5: server step scheduling")
     ap.add_argument("--requests", type=int, default=200, help="c5: number of requests")
@@ -832,7 +911,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     line = {"c2": bench_c2, "c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
-            "copy": bench_copy, "sweep": sweep_c2, "bounce": bench_bounce}[args.config](args)
+            "copy": bench_copy, "sweep": sweep_c2, "bounce": bench_bounce, "reference": bench_reference}[args.config](args)
     print(json.dumps(line), flush=True)
 
 
