@@ -125,10 +125,8 @@ void ensure_wave_buffers(Context& ctx, Plan& pl)
     const size_t nslots = (size_t)pl.args.ntrace * 512;
     pl.hits = dev_alloc(ctx, nslots * sizeof(int4));
     pl.vis = dev_alloc(ctx, std::max<size_t>(16, nslots * (size_t)pl.args.nvis));
-    pl.wctr = dev_alloc(ctx, 16 * sizeof(unsigned long long));
     pl.args.hits = pl.hits.as<int4>();
     pl.args.vis = pl.vis.as<uint8_t>();
-    pl.args.wctr = pl.wctr.as<unsigned long long>();
 }
 
 // Applies a trace_variant to a plan (launch shapes and lane map).
@@ -214,6 +212,7 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     pl.out = dev_alloc(ctx, (size_t)P * K * 3 * sizeof(float));
     pl.out64 = dev_alloc(ctx, (size_t)P * K * 3 * sizeof(double));
     pl.counters = dev_alloc(ctx, 4 * sizeof(unsigned long long));
+    pl.wctr = dev_alloc(ctx, 16 * sizeof(unsigned long long));  // work counters (every variant)
 
     // ---- kernel arguments
     BakeArgs& a = pl.args;
@@ -243,6 +242,7 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     a.probe_perm = pl.ps.probe_perm.as<int>();
     a.dir_perm = pl.rt.dir_perm.as<int>();
     a.counters = pl.counters.as<unsigned long long>();
+    a.wctr = pl.wctr.as<unsigned long long>();
 
     MergeArgs& m = pl.margs;
     m.partials = pl.partials.as<float2>();

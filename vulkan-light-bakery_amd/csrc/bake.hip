@@ -22,15 +22,25 @@
 //
 // k_trace walks its rays XCD by XCD: workgroup b runs on XCD b % 8
 // (round-robin dispatch), and XCD x owns the x-th contiguous eighth of the
-// rays, so neighbouring probes and directions share that XCD's L2.
+// rays, so neighbouring probes and directions share that XCD's L2.  Work is
+// taken 256 rays at a time from the XCD's counter, then from the other
+// XCDs' once it is drained, so the launch ends when the work does rather
+// than when its unluckiest workgroup does.
 #include "project_common.hpp"
 
 namespace vlb {
 namespace {
 
+// Clears the eight per-XCD work counters (a kernel, not a memset, so that
+// captured graphs replay it).
+__global__ void k_zero_ctr8(unsigned long long* c)
+{
+    if (threadIdx.x < 8) c[threadIdx.x] = 0ull;
+}
+
 // ----------------------------------------------------------- trace + shade
-// One ray per lane, persistent grid-stride over "trace tiles" (16 probes x
-// 32 rays = 512 rays).  Thread t of a tile traces for probe 16*pg + (l & 15)
+// One ray per lane, persistent, over "trace tiles" (16 probes x 32 rays =
+// 512 rays) taken half a tile per workgroup and fetch.  Thread t of a tile traces for probe 16*pg + (l & 15)
 // the ray 32*ch + 8*(l >> 4) + e, with l = t >> 3 and e = t & 7, and stores
 // its radiance at rad[((tile*3 + c)*64 + l)*8 + e]: the 16x16 MFMA A-operand
 // order of the projection kernel, written fully coalesced.  The lane map
@@ -42,6 +52,7 @@ template <bool RECORD, bool COUNT, int OCC, bool IND>
 __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
 {
     __shared__ uint32_t lds[kLdsDepth * 256];
+    __shared__ unsigned long long fetched;
     const int tid = threadIdx.x;
     Stack st;
     st.lds = &lds[tid];
@@ -49,13 +60,35 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
     st.spill_stride = a.spill_stride;
     TraceCount cnt;
     uint32_t nshadow = 0;
-    // XCD x (= blockIdx % 8) owns the x-th contiguous eighth of the rays
+    // eight 256-aligned ranges, XCD x (= blockIdx % 8) owning the x-th;
+    // nitems is a multiple of 512, so a fetched block of 256 is whole
     const size_t nitems = (size_t)a.ntrace * 512;
-    const size_t per_xcd = (nitems + 7) >> 3;
-    const size_t xbeg = (size_t)(blockIdx.x & 7) * per_xcd;
-    const size_t xend = min(nitems, xbeg + per_xcd);
-    const size_t stride = (size_t)(gridDim.x >> 3) * 256;
-    for (size_t it = xbeg + (size_t)(blockIdx.x >> 3) * 256 + tid; it < xend; it += stride) {
+    const size_t per_xcd = ((nitems + 7) / 8 + 255) / 256 * 256;
+    const int xcd = blockIdx.x & 7;
+    int victim = 0;  // thread 0 only: ranges tried, own first
+    for (;;) {
+        if (tid == 0) {
+            unsigned long long b = ~0ull;
+            while (victim < 8) {
+                const int v = (xcd + victim) & 7;
+                const size_t vbeg = (size_t)v * per_xcd;
+                const size_t vend = min(nitems, vbeg + per_xcd);
+                if (vbeg < vend) {
+                    const unsigned long long base = atomicAdd(&a.wctr[v], 256ull);
+                    if (vbeg + base < vend) {
+                        b = vbeg + base;
+                        break;
+                    }
+                }
+                ++victim;
+            }
+            fetched = b;
+        }
+        __syncthreads();
+        const unsigned long long b = fetched;
+        __syncthreads();
+        if (b == ~0ull) break;
+        const size_t it = (size_t)b + tid;
         int tile, l, e;
         tile_map(a.map, it, tile, l, e);
         const int t = l * 8 + e;
@@ -198,6 +231,7 @@ int trace_grid_size(Context& ctx, int occ)
 void launch_trace(const BakeArgs& a, bool record, bool count, int occ, int grid, hipStream_t st)
 {
     VLB_REQUIRE(grid % 8 == 0, VLB_ERR_INVALID, "trace grid must be a multiple of 8");
+    k_zero_ctr8<<<1, 64, 0, st>>>(a.wctr);
     if (a.sp.ind.coeffs) {
         if (record) k_trace<true, false, 0, true><<<grid, 256, 0, st>>>(a);
         else if (count) k_trace<false, true, 0, true><<<grid, 256, 0, st>>>(a);
