@@ -171,9 +171,8 @@ typedef struct vlb_bake_params {
                                       /*   shadow, shade): 13 + map +        */
                                       /*   4*refill, refill {16, 32, 8}      */
                                       /*   idle lanes; 0 = autotuned choice, */
-                                      /*   else 12 below 2^24 primary rays,  */
-                                      /*   20 from there on or with an       */
-                                      /*   indirect term                     */
+                                      /*   else 12 (20 with an indirect      */
+                                      /*   term)                             */
     uint32_t         reserved[2];
 } vlb_bake_params;
 

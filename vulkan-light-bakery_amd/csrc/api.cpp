@@ -82,17 +82,15 @@ namespace {
 // 13..24 wavefront k_rays/k_shade (13 + map + 4 * refill index)
 constexpr uint32_t kFusedDefault = 12;  // fused, lane map 3, 8 waves/SIMD
 constexpr uint32_t kWaveDefault = 20;   // wavefront, lane map 3, refill at 32 idle lanes
-constexpr uint64_t kWaveFromRays = 1ull << 24;
 
-// trace_variant of a plan that leaves it to the library.  Below 2^24 primary
-// rays the fused kernel wins (the wavefront pipeline's three passes and its
-// hit-record traffic are a fixed cost); from there on the wavefront kernels'
-// refill of idle lanes wins, by up to 1.5x on 0.5-2M triangle scenes at
-// 2^28 rays.  With an indirect term the eight corner visibility rays per hit
-// serialise in a fused lane, so the wavefront kernels are used at any size.
-uint32_t default_variant(uint64_t probes, uint64_t rays, bool indirect)
+// trace_variant of a plan that leaves it to the library: the fused kernel,
+// best or within ~10% of the best on rooms, terrains and a city block of
+// 11k-2M triangles at 256-65,536 probes; with an indirect term the eight
+// corner visibility rays per hit serialise inside a fused lane, and the
+// wavefront kernels are faster.
+uint32_t default_variant(bool indirect)
 {
-    return indirect || probes * rays >= kWaveFromRays ? kWaveDefault : kFusedDefault;
+    return indirect ? kWaveDefault : kFusedDefault;
 }
 constexpr uint32_t kMaxVariant = 24;
 constexpr int kOccTab[3] = {0, 6, 8};
@@ -184,7 +182,7 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     if (tv == 0) {
         auto it = ctx.tuned.find({sc.id, R});
         pl.default_variant = it == ctx.tuned.end();
-        tv = pl.default_variant ? default_variant(p.num_probes, R, false) : it->second;
+        tv = pl.default_variant ? default_variant(false) : it->second;
     }
     VLB_REQUIRE(tv >= 1 && tv <= kMaxVariant, VLB_ERR_INVALID, "trace_variant out of range");
     pl.grid = trace_grid_size(ctx, 0);
@@ -718,7 +716,7 @@ vlb_status vlb_plan_set_indirect(vlb_plan* h, const vlb_indirect_params* ip, con
             memset(&a.sp.ind, 0, sizeof a.sp.ind);
             if (pl->params.shade_mode == VLB_SHADE_LAMBERT) a.nvis = nvis0;
             pl->indirect = false;
-            if (pl->default_variant) apply_variant(ctx, *pl, default_variant(a.P, a.R, false));
+            if (pl->default_variant) apply_variant(ctx, *pl, default_variant(false));
             drop_graph(*pl);
             return;
         }
@@ -737,7 +735,7 @@ vlb_status vlb_plan_set_indirect(vlb_plan* h, const vlb_indirect_params* ip, con
         else launch_copy_f32(pl->ind.as<float>(), pl->out.as<float>(), n, ctx.stream);
         const int nvis = nvis0 + (ip->lookup.visibility ? 8 : 0);
         a.nvis = nvis;
-        if (pl->default_variant) apply_variant(ctx, *pl, default_variant(a.P, a.R, true));
+        if (pl->default_variant) apply_variant(ctx, *pl, default_variant(true));
         if (pl->hits.p && pl->vis.bytes < (size_t)a.ntrace * 512 * (size_t)nvis) {
             pl->vis = dev_alloc(ctx, (size_t)a.ntrace * 512 * (size_t)nvis);
             a.vis = pl->vis.as<uint8_t>();
