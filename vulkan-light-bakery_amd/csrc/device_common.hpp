@@ -561,18 +561,12 @@ __device__ float3 grid_indirect(const SceneView& sc, const ProbeGridView& g, con
     return make_float3(g.scale * (acc[0] / sw), g.scale * (acc[1] / sw), g.scale * (acc[2] / sw));
 }
 
-// Lambert term of light j at hit point P with the two-sided normal N: the
-// contribution E cos is formed before the shadow query, so only it (not E
-// and the cosine) is live across the traversal; added to acc when unoccluded.
-template <bool COUNT>
-__device__ __forceinline__ void light_term(const SceneView& sc, const ShadeParams& sp, int j,
-                                           const float3& P, const float3& N, float3& acc,
-                                           const Stack& st, TraceCount& cnt, uint32_t& nshadow)
+// Direction Ld and distance of light j seen from hit point P, and its
+// irradiance E there (before the cosine).
+__device__ __forceinline__ void light_geom(const ShadeParams& sp, int j, const float3& P,
+                                           float3& Ld, float& dist, float3& E)
 {
     const vlb_light& Lj = sp.lights[j];
-    float3 Ld;
-    float dist;
-    float3 E;
     if (Lj.type == VLB_LIGHT_POINT) {
         const float3 Lv = make_float3(Lj.vec[0] - P.x, Lj.vec[1] - P.y, Lj.vec[2] - P.z);
         dist = sqrtf(dot3(Lv, Lv));
@@ -585,11 +579,30 @@ __device__ __forceinline__ void light_term(const SceneView& sc, const ShadeParam
         dist = sp.t_max;
         E = make_float3(Lj.rgb[0], Lj.rgb[1], Lj.rgb[2]);
     }
+}
+
+// Origin of the shadow ray from hit point P with the two-sided normal N.
+__device__ __forceinline__ float3 shadow_origin(const ShadeParams& sp, const float3& P, const float3& N)
+{
+    return make_float3(P.x + sp.shadow_bias * N.x, P.y + sp.shadow_bias * N.y,
+                       P.z + sp.shadow_bias * N.z);
+}
+
+// Lambert term of light j at hit point P with the two-sided normal N: the
+// contribution E cos is formed before the shadow query, so only it (not E
+// and the cosine) is live across the traversal; added to acc when unoccluded.
+template <bool COUNT>
+__device__ __forceinline__ void light_term(const SceneView& sc, const ShadeParams& sp, int j,
+                                           const float3& P, const float3& N, float3& acc,
+                                           const Stack& st, TraceCount& cnt, uint32_t& nshadow)
+{
+    float3 Ld, E;
+    float dist;
+    light_geom(sp, j, P, Ld, dist, E);
     const float cosNL = dot3(N, Ld);
     if (cosNL > 0.0f) {
         const float3 c = make_float3(E.x * cosNL, E.y * cosNL, E.z * cosNL);
-        const float3 org = make_float3(P.x + sp.shadow_bias * N.x, P.y + sp.shadow_bias * N.y,
-                                       P.z + sp.shadow_bias * N.z);
+        const float3 org = shadow_origin(sp, P, N);
         ++nshadow;
         if (!trace_any<COUNT>(sc, org, Ld, 0.0f, dist, st, cnt)) {
             acc.x = acc.x + c.x;
@@ -597,6 +610,24 @@ __device__ __forceinline__ void light_term(const SceneView& sc, const ShadeParam
             acc.z = acc.z + c.z;
         }
     }
+}
+
+// Hit point and interpolated shading normal (env_map.rchit:63-68).
+__device__ __forceinline__ void hit_point_normal(const SceneView& sc, int prim, const float3& o,
+                                                 const float3& d, float t, float u, float v,
+                                                 float3& P, float3& N)
+{
+    const float4 s0 = sc.shade[3 * prim + 0];
+    const float4 s1 = sc.shade[3 * prim + 1];
+    const float4 s2 = sc.shade[3 * prim + 2];
+    const float b0 = 1.0f - u - v;
+    P = make_float3(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+    float3 m;
+    m.x = s0.x * b0 + s1.x * u + s2.x * v;
+    m.y = s0.y * b0 + s1.y * u + s2.y * v;
+    m.z = s0.z * b0 + s1.z * u + s2.z * v;
+    const float ml = sqrtf(dot3(m, m));
+    N = ml > 0.0f ? make_float3(m.x / ml, m.y / ml, m.z / ml) : make_float3(0.f, 0.f, 0.f);
 }
 
 // Radiance carried by one primary ray.  `nshadow` counts traced shadow rays.
@@ -627,18 +658,8 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
         }
         return s;
     }
-    // env_map.rchit:63-68
-    const float4 s0 = sc.shade[3 * prim + 0];
-    const float4 s1 = sc.shade[3 * prim + 1];
-    const float4 s2 = sc.shade[3 * prim + 2];
-    const float b0 = 1.0f - u - v;
-    const float3 P = make_float3(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
-    float3 m;
-    m.x = s0.x * b0 + s1.x * u + s2.x * v;
-    m.y = s0.y * b0 + s1.y * u + s2.y * v;
-    m.z = s0.z * b0 + s1.z * u + s2.z * v;
-    const float ml = sqrtf(dot3(m, m));
-    float3 N = ml > 0.0f ? make_float3(m.x / ml, m.y / ml, m.z / ml) : make_float3(0.f, 0.f, 0.f);
+    float3 P, N;
+    hit_point_normal(sc, prim, o, d, t, u, v, P, N);
     if (ref) {
         // env_map.rchit:73-101, main.rchit:95-116
         const vlb_ref_shading& r = sp.ref;
