@@ -31,11 +31,14 @@
 namespace vlb {
 namespace {
 
-// Clears the eight per-XCD work counters (a kernel, not a memset, so that
-// captured graphs replay it).
+// The eight per-XCD work counters of the fused kernel sit 128 bytes apart,
+// one per cache line: device-scope atomics serialise per line.
+constexpr int kCtrStride = 16;
+
+// Clears them (a kernel, not a memset, so that captured graphs replay it).
 __global__ void k_zero_ctr8(unsigned long long* c)
 {
-    if (threadIdx.x < 8) c[threadIdx.x] = 0ull;
+    if (threadIdx.x < 8) c[threadIdx.x * kCtrStride] = 0ull;
 }
 
 // ----------------------------------------------------------- trace + shade
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
                 const size_t vbeg = (size_t)v * per_xcd;
                 const size_t vend = min(nitems, vbeg + per_xcd);
                 if (vbeg < vend) {
-                    const unsigned long long base = atomicAdd(&a.wctr[v], 256ull);
+                    const unsigned long long base = atomicAdd(&a.wctr[v * kCtrStride], 256ull);
                     if (vbeg + base < vend) {
                         b = vbeg + base;
                         break;
