@@ -31,14 +31,14 @@
 namespace vlb {
 namespace {
 
-// The eight per-XCD work counters of the fused kernel sit 128 bytes apart,
-// one per cache line: device-scope atomics serialise per line.
+// The 32 work counters of the fused kernel sit 128 bytes apart, one per
+// cache line: device-scope atomics serialise per line.
 constexpr int kCtrStride = 16;
 
 // Clears them (a kernel, not a memset, so that captured graphs replay it).
 __global__ void k_zero_ctr8(unsigned long long* c)
 {
-    if (threadIdx.x < 8) c[threadIdx.x * kCtrStride] = 0ull;
+    if (threadIdx.x < 32) c[threadIdx.x * kCtrStride] = 0ull;
 }
 
 // ----------------------------------------------------------- trace + shade
@@ -55,29 +55,32 @@ template <bool RECORD, bool COUNT, int OCC, bool IND>
 __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
 {
     __shared__ uint32_t lds[kLdsDepth * 256];
-    __shared__ unsigned long long fetched;
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
     Stack st;
     st.lds = &lds[tid];
     st.spill = a.spill + (size_t)blockIdx.x * 256 + tid;
     st.spill_stride = a.spill_stride;
     TraceCount cnt;
     uint32_t nshadow = 0;
-    // eight 256-aligned ranges, XCD x (= blockIdx % 8) owning the x-th;
-    // nitems is a multiple of 512, so a fetched block of 256 is whole
+    // 32 64-aligned ranges, range 4x + w owned by wave w of the workgroups on
+    // XCD x (= blockIdx % 8); a wave takes 64 rays per fetch from its own
+    // range, then from its XCD's other ranges, then from the other XCDs'
+    // (nitems is a multiple of 512, so a fetched chunk is whole)
     const size_t nitems = (size_t)a.ntrace * 512;
-    const size_t per_xcd = ((nitems + 7) / 8 + 255) / 256 * 256;
-    const int xcd = blockIdx.x & 7;
-    int victim = 0;  // thread 0 only: ranges tried, own first
+    const size_t per_rng = ((nitems + 31) / 32 + 63) / 64 * 64;
+    const int own = (blockIdx.x & 7) * 4 + (tid >> 6);
+    int victim = 0;  // ranges tried (wave-uniform)
     for (;;) {
-        if (tid == 0) {
-            unsigned long long b = ~0ull;
-            while (victim < 8) {
-                const int v = (xcd + victim) & 7;
-                const size_t vbeg = (size_t)v * per_xcd;
-                const size_t vend = min(nitems, vbeg + per_xcd);
+        unsigned long long b = ~0ull;
+        if (lane == 0) {
+            while (victim < 32) {
+                const int v = (own & ~3) + ((own + victim) & 3) + 4 * (victim >> 2);
+                const int vv = v & 31;
+                const size_t vbeg = (size_t)vv * per_rng;
+                const size_t vend = min(nitems, vbeg + per_rng);
                 if (vbeg < vend) {
-                    const unsigned long long base = atomicAdd(&a.wctr[v * kCtrStride], 256ull);
+                    const unsigned long long base = atomicAdd(&a.wctr[vv * kCtrStride], 64ull);
                     if (vbeg + base < vend) {
                         b = vbeg + base;
                         break;
@@ -85,13 +88,11 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
                 }
                 ++victim;
             }
-            fetched = b;
         }
-        __syncthreads();
-        const unsigned long long b = fetched;
-        __syncthreads();
+        b = ((unsigned long long)__shfl((unsigned)(b >> 32), 0) << 32) | (unsigned long long)__shfl((unsigned)b, 0);
+        victim = __shfl(victim, 0);
         if (b == ~0ull) break;
-        const size_t it = (size_t)b + tid;
+        const size_t it = (size_t)b + lane;
         int tile, l, e;
         tile_map(a.map, it, tile, l, e);
         const int t = l * 8 + e;
