@@ -212,7 +212,7 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
     pl.counters = dev_alloc(ctx, 4 * sizeof(unsigned long long));
     // work counters of every variant: 8 per-XCD counters 128 B apart (fused),
     // or 16 packed ones (wavefront)
-    pl.wctr = dev_alloc(ctx, 512 * sizeof(unsigned long long));
+    pl.wctr = dev_alloc(ctx, 32 * kCtrStride * sizeof(unsigned long long));
 
     // ---- kernel arguments
     BakeArgs& a = pl.args;

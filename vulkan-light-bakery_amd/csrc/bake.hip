@@ -31,11 +31,7 @@
 namespace vlb {
 namespace {
 
-// The 32 work counters of the fused kernel sit 128 bytes apart, one per
-// cache line: device-scope atomics serialise per line.
-constexpr int kCtrStride = 16;
-
-// Clears them (a kernel, not a memset, so that captured graphs replay it).
+// Clears the 32 work counters of the fused kernel (a kernel, not a memset, so that captured graphs replay it).
 __global__ void k_zero_ctr8(unsigned long long* c)
 {
     if (threadIdx.x < 32) c[threadIdx.x * kCtrStride] = 0ull;

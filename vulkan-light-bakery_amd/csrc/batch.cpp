@@ -197,7 +197,7 @@ void RoundBufs::alloc(Context& ctx, size_t cap, int nv, uint32_t dep)
     rad = dev_alloc(ctx, cap * 1536 * sizeof(float));
     hits = dev_alloc(ctx, cap * 512 * sizeof(int4));
     vis = dev_alloc(ctx, std::max<size_t>(16, cap * 512 * (size_t)nv));
-    wctr = dev_alloc(ctx, 16 * sizeof(unsigned long long));
+    wctr = dev_alloc(ctx, 32 * kCtrStride * sizeof(unsigned long long));
     counters = dev_alloc(ctx, 4 * sizeof(unsigned long long));
     const uint32_t sb = stack_bound(dep);
     const uint32_t e = sb > (uint32_t)kLdsDepth ? sb - kLdsDepth : 0;

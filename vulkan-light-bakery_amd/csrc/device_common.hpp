@@ -96,6 +96,9 @@ struct ShadeParams {
 };
 
 constexpr int kLdsDepth = 16;  // stack entries kept in LDS per lane
+// Work counters (persistent kernels' fetch positions) sit 128 bytes apart,
+// one per cache line: device-scope atomics on one line serialise.
+constexpr int kCtrStride = 16;
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ float dot3(const float3& a, const float3& b)

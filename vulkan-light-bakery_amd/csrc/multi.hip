@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256) void k_rays_jobs(MultiArgs a, const JobDesc* _
     const int nseg = hdr->nseg;
     const size_t nslots = (size_t)hdr->ntrace * 512;
     const size_t nitems = ANY ? nslots * (size_t)min(hdr->nvis, a.nvis) : nslots;
-    unsigned long long* ctr = a.wctr + (ANY ? 8 : 0);
+    unsigned long long* ctr = a.wctr + (ANY ? 8 * kCtrStride : 0);
     const size_t chunk = (size_t)a.chunk;
     // eight chunk-aligned ranges, one per XCD
     const size_t per_xcd = ((nitems + 7) / 8 + chunk - 1) / chunk * chunk;
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void k_rays_jobs(MultiArgs a, const JobDesc* _
                         const size_t vend = min(nitems, vbeg + per_xcd);
                         unsigned long long base = 0;
                         if (vbeg < vend && lane == 0)
-                            base = atomicAdd(&ctr[v], (unsigned long long)chunk);
+                            base = atomicAdd(&ctr[v * kCtrStride], (unsigned long long)chunk);
                         base = __shfl(base, 0);
                         if (vbeg < vend && vbeg + base < vend) {
                             pnext = vbeg + base;
@@ -260,8 +260,8 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_fused_jobs(MultiArgs
                                                                        const WorkHeader* __restrict__ hdr)
 {
     __shared__ uint32_t lds[kLdsDepth * 256];
-    __shared__ unsigned long long fetched;
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
     Stack st;
     st.lds = &lds[tid];
     st.spill = a.spill + (size_t)blockIdx.x * 256 + tid;
@@ -270,19 +270,22 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_fused_jobs(MultiArgs
     uint32_t nshadow = 0;
     const int nseg = hdr->nseg;
     const size_t nitems = (size_t)hdr->ntrace * 512;
-    // eight 256-aligned ranges, one per XCD (nitems is a multiple of 512)
-    const size_t per_xcd = ((nitems + 7) / 8 + 255) / 256 * 256;
-    const int xcd = blockIdx.x & 7;
-    int victim = 0;  // thread 0 only: ranges tried, own first
+    // 32 64-aligned ranges, range 4x + w owned by wave w of the workgroups on
+    // XCD x; a wave takes 64 items per fetch from its own range, then from
+    // its XCD's other ranges, then from the other XCDs' (nitems is a multiple
+    // of 512, so a fetched chunk is whole and lies in one pair of tiles)
+    const size_t per_rng = ((nitems + 31) / 32 + 63) / 64 * 64;
+    const int own = (blockIdx.x & 7) * 4 + (tid >> 6);
+    int victim = 0;  // ranges tried (wave-uniform)
     for (;;) {
-        if (tid == 0) {
-            unsigned long long b = ~0ull;
-            while (victim < 8) {
-                const int v = (xcd + victim) & 7;
-                const size_t vbeg = (size_t)v * per_xcd;
-                const size_t vend = min(nitems, vbeg + per_xcd);
+        unsigned long long b = ~0ull;
+        if (lane == 0) {
+            while (victim < 32) {
+                const int v = ((own & ~3) + ((own + victim) & 3) + 4 * (victim >> 2)) & 31;
+                const size_t vbeg = (size_t)v * per_rng;
+                const size_t vend = min(nitems, vbeg + per_rng);
                 if (vbeg < vend) {
-                    const unsigned long long base = atomicAdd(&a.wctr[v], 256ull);
+                    const unsigned long long base = atomicAdd(&a.wctr[v * kCtrStride], 64ull);
                     if (vbeg + base < vend) {
                         b = vbeg + base;
                         break;
@@ -290,13 +293,11 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_fused_jobs(MultiArgs
                 }
                 ++victim;
             }
-            fetched = b;
         }
-        __syncthreads();
-        const unsigned long long b = fetched;
-        __syncthreads();
+        b = ((unsigned long long)__shfl((unsigned)(b >> 32), 0) << 32) | (unsigned long long)__shfl((unsigned)b, 0);
+        victim = __shfl(victim, 0);
         if (b == ~0ull) break;
-        const size_t it = (size_t)b + tid;
+        const size_t it = (size_t)b + lane;
         int tile, l, e;
         tile_map(a.map, it, tile, l, e);
         const int tile0 = __builtin_amdgcn_readfirstlane((int)(b >> 10) * 2);
@@ -415,10 +416,11 @@ __global__ __launch_bounds__(256) void k_merge_jobs(MultiArgs a, const JobDesc* 
     }
 }
 
+// Clears the first n work counters (128 bytes apart).
 __global__ void k_zero_ctr(unsigned long long* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0ull;
+    if (i < n) p[(size_t)i * kCtrStride] = 0ull;
 }
 
 template <class K>
@@ -452,7 +454,7 @@ void launch_multi_trace(const MultiArgs& a, bool count, int grid, int shade_grid
     VLB_REQUIRE(grid % 8 == 0 && grid > 0, VLB_ERR_INVALID, "multi-job grid must be a multiple of 8");
     VLB_REQUIRE(a.chunk >= 64 && a.chunk <= 1024 && (1024 % a.chunk) == 0, VLB_ERR_INVALID,
                 "work chunk must divide 1024");
-    k_zero_ctr<<<1, 64, 0, st>>>(a.wctr, 16);
+    k_zero_ctr<<<1, 64, 0, st>>>(a.wctr, 32);
     if (a.focc >= 0) {
         VLB_REQUIRE(a.fgrid % 8 == 0 && a.fgrid > 0, VLB_ERR_INVALID, "fused grid must be a multiple of 8");
         if (count) k_fused_jobs<true, 0><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);

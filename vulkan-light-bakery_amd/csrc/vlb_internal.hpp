@@ -215,7 +215,7 @@ struct BakeArgs {
     uint8_t* vis;             // [slot][nvis] shadow-ray occlusion
     int nvis;                 // shadow rays per primary ray (lights)
     int refill;               // idle lanes that trigger a refill (1..64)
-    unsigned long long* wctr; // 16 work counters (per XCD, primary + shadow)
+    unsigned long long* wctr; // 32 work counters, kCtrStride apart
     float tmin, tmax;
     float2* partials;         // [nslices][Ppad][3][16] (sum, compensation)
     uint32_t* spill;
@@ -334,7 +334,7 @@ struct MultiArgs {
     int map;                  // lane -> ray map (0..3)
     int refill;               // idle lanes that trigger a refill
     int chunk;                // items per work fetch (64..1024, divides 1024)
-    unsigned long long* wctr; // 16 work counters: [0..7] primary, [8..15] shadow
+    unsigned long long* wctr; // 32 work counters, kCtrStride apart
     uint32_t* spill;
     uint32_t spill_stride;
     unsigned long long* counters;  // node visits, tri tests, shadow rays (count mode)

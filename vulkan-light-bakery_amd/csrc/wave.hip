@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigne
             while (idle != 0 && !drained) {
                 if (pnext >= pend) {
                     unsigned long long base = 0;
-                    if (lane == 0) base = atomicAdd(&ctr[xcd], (unsigned long long)kChunk);
+                    if (lane == 0) base = atomicAdd(&ctr[xcd * kCtrStride], (unsigned long long)kChunk);
                     base = __shfl(base, 0);
                     pnext = xbeg + base;
                     pend = min(xend, pnext + kChunk);
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void k_shade(BakeArgs a)
 __global__ void k_zero_u64(unsigned long long* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0ull;
+    if (i < n) p[(size_t)i * kCtrStride] = 0ull;
 }
 
 template <class K>
@@ -265,9 +265,9 @@ void launch_wave(const BakeArgs& a, bool record, bool count, int grid, hipStream
 {
     VLB_REQUIRE(grid % 8 == 0, VLB_ERR_INVALID, "wave grid must be a multiple of 8");
     const size_t nslots = (size_t)a.ntrace * 512;
-    // work counters: [0..7] primary, [8..15] shadow (cleared by a kernel so
-    // the sequence can be captured in a graph)
-    k_zero_u64<<<1, 64, 0, st>>>(a.wctr, 16);
+    // work counters 0..7 primary, 8..15 shadow, kCtrStride apart (cleared by
+    // a kernel so the sequence can be captured in a graph)
+    k_zero_u64<<<1, 64, 0, st>>>(a.wctr, 16);  // primary and shadow counters
     if (count) k_rays<false, true, false><<<grid, 256, 0, st>>>(a, nslots, a.wctr);
     else k_rays<false, false, false><<<grid, 256, 0, st>>>(a, nslots, a.wctr);
     VLB_HIP(hipGetLastError());
@@ -275,12 +275,12 @@ void launch_wave(const BakeArgs& a, bool record, bool count, int grid, hipStream
     if (a.nvis > 0) {
         const bool corners = a.sp.ind.coeffs && a.sp.ind.vis;
         if (corners) {
-            if (count) k_rays<true, true, true><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8);
-            else k_rays<true, false, true><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8);
+            if (count) k_rays<true, true, true><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8 * kCtrStride);
+            else k_rays<true, false, true><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8 * kCtrStride);
         } else if (count) {
-            k_rays<true, true, false><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8);
+            k_rays<true, true, false><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8 * kCtrStride);
         } else {
-            k_rays<true, false, false><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8);
+            k_rays<true, false, false><<<grid, 256, 0, st>>>(a, nshadow, a.wctr + 8 * kCtrStride);
         }
         VLB_HIP(hipGetLastError());
     }
