@@ -11,8 +11,9 @@
 //                  order -> radiance in the MFMA operand order of k_project
 // Both traversal kernels are persistent with dynamic ray fetch: when at
 // least `refill` lanes of a wave are idle, the wave hands them the next rays
-// of its XCD's work range (one atomic per chunk of rays), so lanes stay busy
-// until the range is drained.  Items that need no traversal (padding, shadow
+// of its XCD's work range (one atomic per chunk of rays), then of the other
+// XCDs' ranges once its own is drained, so lanes stay busy until all the
+// work is taken.  Items that need no traversal (padding, shadow
 // pairs with N.L <= 0 or a missed primary) are retired at fetch time.
 //
 // Determinism: every decision (hit, N.L > 0, occlusion) is a function of the
@@ -137,13 +138,14 @@ __global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigne
     st.spill_stride = a.spill_stride;
     TraceCount cnt;
     uint32_t ntraced = 0;
-    // this XCD's share of the item space
+    // eight kChunk-aligned ranges of the item space, one per XCD; a wave
+    // takes chunks from its XCD's range, then from the others' once that is
+    // drained, so the launch ends when the work does
     const int xcd = blockIdx.x & 7;
-    const size_t per_xcd = (nitems + 7) >> 3;
-    const size_t xbeg = (size_t)xcd * per_xcd;
-    const size_t xend = min(nitems, xbeg + per_xcd);
+    const size_t per_xcd = ((nitems + 7) / 8 + kChunk - 1) / kChunk * kChunk;
+    int victim = 0;              // ranges tried: own first, then xcd+1, ...
     size_t pnext = 0, pend = 0;  // wave's current chunk [pnext, pend)
-    bool drained = xbeg >= xend;
+    bool drained = nitems == 0;
 
     TravState T;
     int slot = 0, light = 0;
@@ -154,15 +156,25 @@ __global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigne
         if (!drained && (int)__popcll(idle) >= a.refill) {
             while (idle != 0 && !drained) {
                 if (pnext >= pend) {
-                    unsigned long long base = 0;
-                    if (lane == 0) base = atomicAdd(&ctr[xcd * kCtrStride], (unsigned long long)kChunk);
-                    base = __shfl(base, 0);
-                    pnext = xbeg + base;
-                    pend = min(xend, pnext + kChunk);
-                    if (pnext >= xend) {
-                        drained = true;
-                        break;
+                    for (;;) {
+                        const int v = (xcd + victim) & 7;
+                        const size_t vbeg = (size_t)v * per_xcd;
+                        const size_t vend = min(nitems, vbeg + per_xcd);
+                        unsigned long long base = 0;
+                        if (vbeg < vend && lane == 0)
+                            base = atomicAdd(&ctr[v * kCtrStride], (unsigned long long)kChunk);
+                        base = __shfl(base, 0);
+                        if (vbeg < vend && vbeg + base < vend) {
+                            pnext = vbeg + base;
+                            pend = min(vend, pnext + kChunk);
+                            break;
+                        }
+                        if (++victim == 8) {
+                            drained = true;
+                            break;
+                        }
                     }
+                    if (drained) break;
                 }
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
