@@ -220,13 +220,15 @@ def bench_c1(args):
     out = np.zeros((B, 9, 3), np.float32)
     ctx.download(d_out, out)
     # batch = 1: one skybox per call, synchronised (launch-latency bound)
-    lat, k1 = [], []
+    lat, k1, m1 = [], [], []
     for i in range(50):
         c0 = time.perf_counter()
         ctx.skybox_sh_device(d_in, N, 1, 2, d_out)
         ctx.sync()
         lat.append(time.perf_counter() - c0)
-        k1.append(ctx.skybox_last_ms()[0])
+        pm = ctx.skybox_last_ms()
+        k1.append(pm[0])
+        m1.append(pm[1])
     orc = oracle_leg(args)
     err, cpu = None, None
     if orc is not None:
@@ -250,7 +252,8 @@ def bench_c1(args):
             "config": {"workload": "c1 skybox SH", "batch": B, "face": N},
             "sh_rel_err_vs_fp64": None if err is None else float(err),
             "batch1": {"wall_us_per_skybox": round(1e6 * float(np.median(lat)), 1),
-                       "kernel_us": round(1e3 * float(np.median(k1)), 2), "calls": len(lat)},
+                       "kernel_us": round(1e3 * float(np.median(k1)), 2),
+                       "merge_us": round(1e3 * float(np.median(m1)), 2), "calls": len(lat)},
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None},

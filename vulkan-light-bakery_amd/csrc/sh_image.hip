@@ -134,7 +134,8 @@ __global__ __launch_bounds__(256) void k_skybox(const float4* __restrict__ cubes
         for (int k = 0; k < K * 3; ++k) partial[(size_t)blockIdx.x * K * 3 + k] = acc[k];
 }
 
-// FP64 merge of per-workgroup partials: out[g][v] = sum_b partial[g][b][v].
+// FP64 merge of per-workgroup partials: out[g][v] = sum_b partial[g][b][v],
+// one thread per output, partials added in order.
 __global__ __launch_bounds__(256) void k_merge_partials(const float* __restrict__ partial,
                                                         int groups, int per_group, int nv,
                                                         float* __restrict__ out,
@@ -147,6 +148,40 @@ __global__ __launch_bounds__(256) void k_merge_partials(const float* __restrict_
     for (int b = 0; b < per_group; ++b) s += (double)partial[((size_t)gi * per_group + b) * nv + v];
     if (out) out[idx] = (float)s;
     if (out64) out64[idx] = s;
+}
+
+// The same merge with one wave per output, for many partials per output
+// (one or a few images split over hundreds of workgroups): lane l adds
+// partials l, l + 64, ... in order, then a fixed butterfly adds the lanes, so
+// the sum is the same on every run.
+__global__ __launch_bounds__(256) void k_merge_partials_wide(const float* __restrict__ partial,
+                                                             int groups, int per_group, int nv,
+                                                             float* __restrict__ out,
+                                                             double* __restrict__ out64)
+{
+    const int idx = (int)((blockIdx.x * 256u + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (idx >= groups * nv) return;  // wave-uniform
+    const int gi = idx / nv, v = idx - gi * nv;
+    double s = 0.0;
+    for (int b = lane; b < per_group; b += 64) s += (double)partial[((size_t)gi * per_group + b) * nv + v];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) {
+        if (out) out[idx] = (float)s;
+        if (out64) out64[idx] = s;
+    }
+}
+
+void launch_merge_partials(const float* partial, int groups, int per_group, int nv, float* out,
+                           double* out64, hipStream_t st)
+{
+    const int n = groups * nv;
+    if (per_group > 64)
+        k_merge_partials_wide<<<(n + 3) / 4, 256, 0, st>>>(partial, groups, per_group, nv, out, out64);
+    else
+        k_merge_partials<<<(n + 255) / 256, 256, 0, st>>>(partial, groups, per_group, nv, out, out64);
+    VLB_HIP(hipGetLastError());
 }
 
 // sh.comp (variant 0) / skybox_sh.comp (variant 1) over an RGBA8 image.
@@ -266,9 +301,7 @@ void launch_skybox_sh(const float4* cubes, int N, int batch, int K, const float*
     VLB_HIP(hipGetLastError());
     if (e1) VLB_HIP(hipEventRecord(e1, st));
     const int nv = K * 3;
-    k_merge_partials<<<(batch * nv + 255) / 256, 256, 0, st>>>(partial, batch, blocks_per_cube, nv,
-                                                                out, nullptr);
-    VLB_HIP(hipGetLastError());
+    launch_merge_partials(partial, batch, blocks_per_cube, nv, out, nullptr, st);
     if (e2) VLB_HIP(hipEventRecord(e2, st));
 }
 
@@ -279,8 +312,7 @@ void launch_equirect_project(const uint8_t* img, int W, int H, int variant, doub
     const int per_block = (n + nblocks - 1) / nblocks;
     k_equirect<<<nblocks, 256, 0, st>>>(img, W, H, variant, per_block, partial);
     VLB_HIP(hipGetLastError());
-    k_merge_partials<<<1, 256, 0, st>>>(partial, 1, nblocks, 48, nullptr, out);
-    VLB_HIP(hipGetLastError());
+    launch_merge_partials(partial, 1, nblocks, 48, nullptr, out, st);
 }
 
 void launch_sh_reconstruct(const float* coeffs, int W, int H, uint8_t* out, hipStream_t st)
