@@ -21,7 +21,10 @@
 // never straddles two segments (chunks divide a pair of trace tiles, and
 // segments hold whole pairs), so the job of a chunk is found once per fetch
 // by a wave-uniform binary search; each lane keeps the BVH pointers of the
-// ray it holds, and lanes of one wave may trace different scenes.
+// ray it holds, and lanes of one wave may trace different scenes.  The fused
+// kernel cuts the item space into 32 ranges (four per XCD, one per wave
+// slot of a workgroup) and fetches 64 items per wave.  Every counter sits on
+// its own cache line: device-scope atomics on one line serialise.
 //
 // Per ray, every decision and every value is computed by the same device
 // functions as the single-bake kernels (wave_common.hpp, project_common.hpp),
@@ -250,9 +253,9 @@ __global__ __launch_bounds__(256) void k_rays_jobs(MultiArgs a, const JobDesc* _
 // Fused form of the three passes above (trace_variant 1..12): each lane takes
 // one primary ray and runs closest hit, shading and its shadow rays with the
 // single-bake kernel's shade_ray, writing radiance in MFMA operand order.  A
-// workgroup fetches 256 items at a time from its XCD's counter and then from
-// the other XCDs' (a chunk lies within a pair of trace tiles, hence one
-// segment and one job).  Same values as the wavefront passes, bit for bit.
+// wave fetches 64 items at a time (a chunk lies within a pair of trace tiles,
+// hence one segment and one job).  Same values as the wavefront passes, bit
+// for bit.
 template <bool COUNT, int OCC>
 __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_fused_jobs(MultiArgs a,
                                                                        const JobDesc* __restrict__ jobs,
