@@ -380,21 +380,6 @@ __device__ __forceinline__ BNode4 load_node_uniform(const QNode4* p)
     return n;
 }
 
-// Fetch of node i: through the scalar cache when the address is the same in
-// every active lane (checked on the address, so lanes of a multi-job wave on
-// different scenes take the vector path), else per lane.
-template <typename NodeT>
-__device__ __forceinline__ BNode4 fetch_node(const NodeT* nodes, uint32_t i)
-{
-    const NodeT* p = nodes + i;
-    const unsigned long long a = (unsigned long long)p;
-    const unsigned long long u =
-        ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
-        (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
-    if (__all(a == u)) return load_node_uniform(reinterpret_cast<const NodeT*>(u));
-    return load_node(nodes, i);
-}
-
 // Child c[k] of a 4-way node with a runtime k, through selects (a register
 // array indexed at run time would be placed in scratch memory).
 __device__ __forceinline__ int pick4(const int c[4], int k)
@@ -433,6 +418,30 @@ __device__ __forceinline__ void slab4(const SlabRay& s, const BNode4& n, float t
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         if (c[k] == kEmpty) e[k] = __builtin_huge_valf();
+}
+
+// Entry distances and ids of the children of node i.  The node comes through
+// the scalar cache when its address is the same in every active lane (checked
+// on the address, so lanes of a multi-job wave on different scenes take the
+// vector path), else per lane.  Each path runs the slab tests on its own
+// registers: merging the two fetches first would copy the 28 scalar node
+// words into vector registers on every uniform visit.
+template <typename NodeT>
+__device__ __forceinline__ void fetch_slab4(const NodeT* nodes, uint32_t i, const SlabRay& s,
+                                            float tmin, float tbest, float e[4], int c[4])
+{
+    const NodeT* p = nodes + i;
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned long long u =
+        ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
+        (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+    if (__all(a == u)) {
+        const BNode4 n = load_node_uniform(reinterpret_cast<const NodeT*>(u));
+        slab4(s, n, tmin, tbest, e, c);
+    } else {
+        const BNode4 n = load_node(nodes, i);
+        slab4(s, n, tmin, tbest, e, c);
+    }
 }
 
 __device__ __forceinline__ void cswap(float& ea, int& ca, float& eb, int& cb)
@@ -508,11 +517,10 @@ __device__ int trace_closest(const SceneView& sc, const float3& o, const float3&
     int sp = 0;
     uint32_t node = 0;
     for (;;) {
-        const BNode4 n = fetch_node(sc.nodes, node);
         if (COUNT) ++cnt.nodes;
         float e[4];
         int c[4];
-        slab4(s, n, tmin, tbest, e, c);
+        fetch_slab4(sc.nodes, node, s, tmin, tbest, e, c);
         // leaves one at a time per lane: the wave runs as many tests as its
         // busiest lane has hit leaves, not one per child slot
         for (uint32_t leaves = take_leaves(e, c); leaves != 0; leaves &= leaves - 1) {
@@ -551,11 +559,10 @@ __device__ bool trace_any(const SceneView& sc, const float3& o, const float3& d,
     int sp = 0;
     uint32_t node = 0;
     for (;;) {
-        const BNode4 n = fetch_node(sc.nodes, node);
         if (COUNT) ++cnt.nodes;
         float e[4];
         int c[4];
-        slab4(s, n, tmin, tmax, e, c);
+        fetch_slab4(sc.nodes, node, s, tmin, tmax, e, c);
         for (uint32_t leaves = take_leaves(e, c); leaves != 0; leaves &= leaves - 1) {
             float3 v0, v1, v2;
             int prim;
