@@ -420,15 +420,16 @@ __device__ __forceinline__ void slab4(const SlabRay& s, const BNode4& n, float t
         if (c[k] == kEmpty) e[k] = __builtin_huge_valf();
 }
 
-// Entry distances and ids of the children of node i.  The node comes through
+// Entry distances and ids of the children of node i, and its hit leaf
+// children as a 4-bit mask (take_leaves).  The node comes through
 // the scalar cache when its address is the same in every active lane (checked
 // on the address, so lanes of a multi-job wave on different scenes take the
 // vector path), else per lane.  Each path runs the slab tests on its own
 // registers: merging the two fetches first would copy the 28 scalar node
 // words into vector registers on every uniform visit.
 template <typename NodeT>
-__device__ __forceinline__ void fetch_slab4(const NodeT* nodes, uint32_t i, const SlabRay& s,
-                                            float tmin, float tbest, float e[4], int c[4])
+__device__ __forceinline__ uint32_t fetch_slab4(const NodeT* nodes, uint32_t i, const SlabRay& s,
+                                                float tmin, float tbest, float e[4], int c[4])
 {
     const NodeT* p = nodes + i;
     const unsigned long long a = (unsigned long long)p;
@@ -438,10 +439,11 @@ __device__ __forceinline__ void fetch_slab4(const NodeT* nodes, uint32_t i, cons
     if (__all(a == u)) {
         const BNode4 n = load_node_uniform(reinterpret_cast<const NodeT*>(u));
         slab4(s, n, tmin, tbest, e, c);
-    } else {
-        const BNode4 n = load_node(nodes, i);
-        slab4(s, n, tmin, tbest, e, c);
+        return take_leaves(e, c);
     }
+    const BNode4 n = load_node(nodes, i);
+    slab4(s, n, tmin, tbest, e, c);
+    return take_leaves(e, c);
 }
 
 __device__ __forceinline__ void cswap(float& ea, int& ca, float& eb, int& cb)
@@ -520,10 +522,10 @@ __device__ int trace_closest(const SceneView& sc, const float3& o, const float3&
         if (COUNT) ++cnt.nodes;
         float e[4];
         int c[4];
-        fetch_slab4(sc.nodes, node, s, tmin, tbest, e, c);
+        const uint32_t hit_leaves = fetch_slab4(sc.nodes, node, s, tmin, tbest, e, c);
         // leaves one at a time per lane: the wave runs as many tests as its
         // busiest lane has hit leaves, not one per child slot
-        for (uint32_t leaves = take_leaves(e, c); leaves != 0; leaves &= leaves - 1) {
+        for (uint32_t leaves = hit_leaves; leaves != 0; leaves &= leaves - 1) {
             float3 v0, v1, v2;
             int prim;
             load_tri(sc.tris, ~pick4(c, __builtin_ctz(leaves)), v0, v1, v2, prim);
@@ -562,8 +564,8 @@ __device__ bool trace_any(const SceneView& sc, const float3& o, const float3& d,
         if (COUNT) ++cnt.nodes;
         float e[4];
         int c[4];
-        fetch_slab4(sc.nodes, node, s, tmin, tmax, e, c);
-        for (uint32_t leaves = take_leaves(e, c); leaves != 0; leaves &= leaves - 1) {
+        const uint32_t hit_leaves = fetch_slab4(sc.nodes, node, s, tmin, tmax, e, c);
+        for (uint32_t leaves = hit_leaves; leaves != 0; leaves &= leaves - 1) {
             float3 v0, v1, v2;
             int prim;
             load_tri(sc.tris, ~pick4(c, __builtin_ctz(leaves)), v0, v1, v2, prim);
