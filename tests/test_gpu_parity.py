@@ -237,8 +237,10 @@ def test_empty_and_single_triangle_scenes(ctx):
 
 
 # batches of 1 (one cubemap per workgroup), 2 and 3 (two per workgroup at
-# L2, the last one alone), power-of-two and other face sizes
-@pytest.mark.parametrize("N,batch", [(32, 2), (32, 1), (24, 3), (17, 5)])
+# L2, the last one alone), power-of-two and other face sizes; faces up to 147
+# texels read each texel's direction from a table, larger ones compute it
+# (147: the largest table, 160: computed)
+@pytest.mark.parametrize("N,batch", [(32, 2), (32, 1), (24, 3), (17, 5), (147, 1), (160, 2)])
 def test_skybox_cubemap_vs_oracle(ctx, N, batch):
     rng = np.random.default_rng(N * batch)
     cubes = []

@@ -442,6 +442,7 @@ vlb_status vlb_ctx_destroy(vlb_ctx* h)
     c->scenes.clear();
     c->user_bufs.clear();
     c->wtab.release();
+    c->sky_tab.release();
     c->sky_partial.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -882,11 +883,21 @@ vlb_status vlb_bake_radiance(vlb_ctx* h, uint32_t scene, const vlb_bake_params* 
 }
 
 // ---------------------------------------------------------- skybox / images
+// Solid-angle table of one face and, for faces whose per-texel direction
+// table fits in 2 MiB (N <= 147, so it stays in every XCD's L2 next to the
+// streamed texels), that table: the projection then reads each texel's
+// direction instead of computing it (the same values, from the same code).
 static void ensure_wtab(Context& c, int N)
 {
     if (c.wtab_n == N && c.wtab.p) return;
     std::vector<float> w = solid_angle_table(N);
     c.wtab = upload_vec(c, w);
+    c.sky_tab.release();
+    const size_t tab_bytes = (size_t)6 * N * N * sizeof(float4);
+    if (tab_bytes <= (2u << 20)) {
+        c.sky_tab = dev_alloc(c, tab_bytes);
+        launch_skybox_table(N, c.wtab.as<float>(), c.sky_tab.as<float4>(), c.stream);
+    }
     c.wtab_n = N;
 }
 
@@ -920,7 +931,7 @@ vlb_status vlb_skybox_sh_d(vlb_ctx* h, const void* d_cubes, uint32_t face_size, 
         const size_t pbytes = (size_t)batch * bpc * K * 3 * sizeof(float);
         if (c->sky_partial.bytes < pbytes) c->sky_partial = dev_alloc(*c, pbytes);
         launch_skybox_sh(static_cast<const float4*>(d_cubes), N, (int)batch, K, c->wtab.as<float>(),
-                         c->sky_partial.as<float>(), bpc, static_cast<float*>(d_out), c->stream,
+                         c->sky_tab.p ? c->sky_tab.as<float4>() : nullptr, c->sky_partial.as<float>(), bpc, static_cast<float*>(d_out), c->stream,
                          c->ev[0], c->ev[1], c->ev[2]);
         c->sky_pending = true;
     });

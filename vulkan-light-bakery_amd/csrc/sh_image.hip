@@ -60,11 +60,9 @@ __device__ __forceinline__ float3 cube_dir(int f, float u, float v)
     return make_float3(d.x * inv, d.y * inv, d.z * inv);
 }
 
-// One texel: SH basis at its direction times its solid angle, into acc.
-template <int K>
-__device__ __forceinline__ void skybox_texel(float* acc, const float4& px, int t, int N, int logN,
-                                             int NN, float s2, float s1,
-                                             const float* __restrict__ wtab)
+// Direction (world frame, unit) and solid angle of texel t of a cube map.
+__device__ __forceinline__ float4 skybox_dir_weight(int t, int N, int logN, int NN, float s2, float s1,
+                                                    const float* __restrict__ wtab)
 {
     int f, rem, j, i;
     if (logN >= 0) {
@@ -81,7 +79,31 @@ __device__ __forceinline__ void skybox_texel(float* acc, const float4& px, int t
     const float u = fmaf((float)i, s2, s1);
     const float v = fmaf((float)j, s2, s1);
     const float3 d = cube_dir(f, u, v);
-    const float w = wtab[rem];
+    return make_float4(d.x, d.y, d.z, wtab[rem]);
+}
+
+// Table of skybox_dir_weight over the 6 N^2 texels of one cube map.
+__global__ __launch_bounds__(256) void k_skybox_table(int N, int logN, const float* __restrict__ wtab,
+                                                      float4* __restrict__ tab)
+{
+    const int NN = N * N;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= 6 * NN) return;
+    tab[t] = skybox_dir_weight(t, N, logN, NN, 2.0f / (float)N, 1.0f / (float)N - 1.0f, wtab);
+}
+
+// One texel: SH basis at its direction times its solid angle, into acc.
+// TAB: direction and weight read from the table (the same values), else
+// computed from the texel index.
+template <int K, bool TAB>
+__device__ __forceinline__ void skybox_texel(float* acc, const float4& px, int t, int N, int logN,
+                                             int NN, float s2, float s1,
+                                             const float* __restrict__ wtab,
+                                             const float4* __restrict__ tab)
+{
+    const float4 dw = TAB ? tab[t] : skybox_dir_weight(t, N, logN, NN, s2, s1, wtab);
+    const float3 d = make_float3(dw.x, dw.y, dw.z);
+    const float w = dw.w;
     const float r = px.x * w, g = px.y * w, b = px.z * w;
     float Y[16];
     sh_exact<K>(d.x, d.z, d.y, Y);  // SH frame: world (x, z, y)
@@ -95,10 +117,11 @@ __device__ __forceinline__ void skybox_texel(float* acc, const float4& px, int t
 
 // Four texels per thread per round, their loads issued before any math so
 // that each lane keeps 64 B in flight.
-template <int K>
+template <int K, bool TAB>
 __global__ __launch_bounds__(256) void k_skybox(const float4* __restrict__ cubes, int N, int logN,
                                                 int per_block, int blocks_per_cube,
                                                 const float* __restrict__ wtab,
+                                                const float4* __restrict__ tab,
                                                 float* __restrict__ partial)
 {
     __shared__ float lds[4 * K * 3];
@@ -122,12 +145,12 @@ __global__ __launch_bounds__(256) void k_skybox(const float4* __restrict__ cubes
             q[u] = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(src + t + 256 * u));
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            skybox_texel<K>(acc, make_float4(q[u][0], q[u][1], q[u][2], q[u][3]), t + 256 * u, N, logN,
-                            NN, s2, s1, wtab);
+            skybox_texel<K, TAB>(acc, make_float4(q[u][0], q[u][1], q[u][2], q[u][3]), t + 256 * u, N,
+                                 logN, NN, s2, s1, wtab, tab);
     }
     for (; t < end; t += 256) {
         const nf4 q = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(src + t));
-        skybox_texel<K>(acc, make_float4(q[0], q[1], q[2], q[3]), t, N, logN, NN, s2, s1, wtab);
+        skybox_texel<K, TAB>(acc, make_float4(q[0], q[1], q[2], q[3]), t, N, logN, NN, s2, s1, wtab, tab);
     }
     block_reduce<K * 3>(acc, lds);
     if (threadIdx.x == 0)
@@ -281,23 +304,38 @@ void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
     VLB_HIP(hipGetLastError());
 }
 
+static int log2_or_minus1(int N)
+{
+    if ((N & (N - 1)) != 0) return -1;
+    int logN = 0;
+    while ((1 << logN) < N) ++logN;
+    return logN;
+}
+
+void launch_skybox_table(int N, const float* wtab, float4* tab, hipStream_t st)
+{
+    const int T = 6 * N * N;
+    k_skybox_table<<<(T + 255) / 256, 256, 0, st>>>(N, log2_or_minus1(N), wtab, tab);
+    VLB_HIP(hipGetLastError());
+}
+
 void launch_skybox_sh(const float4* cubes, int N, int batch, int K, const float* wtab,
-                      float* partial, int blocks_per_cube, float* out, hipStream_t st,
-                      hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
+                      const float4* tab, float* partial, int blocks_per_cube, float* out,
+                      hipStream_t st, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
 {
     const int T = 6 * N * N;
     const int per_block = (T + blocks_per_cube - 1) / blocks_per_cube;
-    int logN = -1;
-    if ((N & (N - 1)) == 0) {
-        logN = 0;
-        while ((1 << logN) < N) ++logN;
-    }
+    const int logN = log2_or_minus1(N);
     if (e0) VLB_HIP(hipEventRecord(e0, st));
     const int grid = batch * blocks_per_cube;
-    if (K == 16)
-        k_skybox<16><<<grid, 256, 0, st>>>(cubes, N, logN, per_block, blocks_per_cube, wtab, partial);
+    if (K == 16 && tab)
+        k_skybox<16, true><<<grid, 256, 0, st>>>(cubes, N, logN, per_block, blocks_per_cube, wtab, tab, partial);
+    else if (K == 16)
+        k_skybox<16, false><<<grid, 256, 0, st>>>(cubes, N, logN, per_block, blocks_per_cube, wtab, tab, partial);
+    else if (tab)
+        k_skybox<9, true><<<grid, 256, 0, st>>>(cubes, N, logN, per_block, blocks_per_cube, wtab, tab, partial);
     else
-        k_skybox<9><<<grid, 256, 0, st>>>(cubes, N, logN, per_block, blocks_per_cube, wtab, partial);
+        k_skybox<9, false><<<grid, 256, 0, st>>>(cubes, N, logN, per_block, blocks_per_cube, wtab, tab, partial);
     VLB_HIP(hipGetLastError());
     if (e1) VLB_HIP(hipEventRecord(e1, st));
     const int nv = K * 3;

@@ -121,6 +121,7 @@ struct Context {
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     int wtab_n = 0;
     DevBuf wtab;
+    DevBuf sky_tab;  // per-texel (direction, solid angle) for small faces, else empty
     DevBuf sky_partial;
     double sky_project_ms = 0.0, sky_merge_ms = 0.0;
     bool sky_pending = false;
@@ -390,7 +391,9 @@ void launch_multi_merge(const MultiArgs& a, int grid, hipStream_t st);
 int  lookup_grid_size(Context& ctx);
 void launch_lookup(const LookupArgs& a, int grid, hipStream_t st);
 // sh_image.hip
+void launch_skybox_table(int N, const float* wtab, float4* tab, hipStream_t st);
 void launch_skybox_sh(const float4* cubes, int N, int batch, int K, const float* wtab,
+                      const float4* tab,
                       float* partial, int blocks_per_cube, float* out, hipStream_t st,
                       hipEvent_t e0, hipEvent_t e1, hipEvent_t e2);
 void launch_equirect_project(const uint8_t* img, int W, int H, int variant, double* out,
