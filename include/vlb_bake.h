@@ -423,7 +423,9 @@ vlb_status  vlb_server_destroy(vlb_server* s);
 /* Skybox SH of float cubemaps: the build's form of skybox_sh.comp
  * (shaders/skybox_sh.comp:23-42, host src/skybox_manager.cpp:107-130) for a
  * 6 x N x N RGBA32F cubemap with exact texel solid angles.  Faces +X,-X,+Y,
- * -Y,+Z,-Z in Vulkan cube-map orientation.  Host in/out. */
+ * -Y,+Z,-Z in Vulkan cube-map orientation.  The context keeps the solid-angle
+ * table of the last face size used and, for N <= 147, a per-texel direction
+ * table (at most 2 MiB) until the next size or vlb_ctx_destroy.  Host in/out. */
 vlb_status  vlb_skybox_sh(vlb_ctx* ctx, const float* cubemaps, uint32_t face_size,
                           uint32_t batch, uint32_t sh_order, float* out, size_t out_len);
 /* Device-resident variant: d_cubemaps and d_out are device pointers from
