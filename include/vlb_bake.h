@@ -171,8 +171,7 @@ typedef struct vlb_bake_params {
                                       /*   shadow, shade): 13 + map +        */
                                       /*   4*refill, refill {16, 32, 8}      */
                                       /*   idle lanes; 0 = autotuned choice, */
-                                      /*   else 12 (20 with an indirect      */
-                                      /*   term)                             */
+                                      /*   else 12                           */
     uint32_t         reserved[2];
 } vlb_bake_params;
 
@@ -504,7 +503,7 @@ vlb_status  vlb_probe_lookup(vlb_ctx* ctx, uint32_t scene, const vlb_probe_grid_
  * coeffs: [P][K][3] host floats to start from, or NULL to start from the
  * plan's last result (the plan must have executed).  ip NULL: back to
  * direct light only.  A plan created with trace_variant 0 and no autotuner
- * pick runs variant 20 while the indirect term is on.
+ * pick runs variant 12 with and without the indirect term.
  * No counterpart call in the reference. */
 typedef struct vlb_indirect_params {
     vlb_probe_grid_desc grid;

@@ -86,9 +86,8 @@ def test_indirect_trace_variants_bitwise(ctx, room):
 
 def test_default_variant_follows_indirect_term(ctx, room):
     """A plan left at trace_variant 0 (and not autotuned) runs the fused
-    kernel (12) for direct light and the wavefront kernels (20) while an
-    indirect term is on; either way the coefficients are the explicit
-    variants' bit for bit."""
+    kernel (12) for direct light and while an indirect term is on; either
+    way the coefficients are the explicit variant's bit for bit."""
     s, sid, o = room
     probes, g = _grid()
     src = o.bake(_params(probes, 640), mode64=True)["coeffs"].astype(np.float32)
@@ -108,13 +107,13 @@ def test_default_variant_follows_indirect_term(ctx, room):
     pl.execute()
     direct = pl.download()
     pl.set_indirect(g, lp, 1.0, coeffs=src)
-    assert pl.stats()["grid"] == g20
+    assert pl.stats()["grid"] == g12
     pl.execute()
     ind = pl.download()
     pl.set_indirect(None)
     assert pl.stats()["grid"] == g12
     pl.destroy()
-    want, _ = _indirect_once(ctx, sid, _params(probes, 640, trace_variant=20), g, lp, src)
+    want, _ = _indirect_once(ctx, sid, _params(probes, 640, trace_variant=12), g, lp, src)
     assert np.array_equal(ind.view(np.uint32), want.view(np.uint32))
     pl = ctx.plan(sid, _params(probes, 640, trace_variant=12))
     pl.execute()

@@ -81,16 +81,16 @@ namespace {
 // trace_variant: 1..12 fused k_trace (1 + map + 4 * occupancy index),
 // 13..24 wavefront k_rays/k_shade (13 + map + 4 * refill index)
 constexpr uint32_t kFusedDefault = 12;  // fused, lane map 3, 8 waves/SIMD
-constexpr uint32_t kWaveDefault = 20;   // wavefront, lane map 3, refill at 32 idle lanes
 
 // trace_variant of a plan that leaves it to the library: the fused kernel,
 // best or within ~10% of the best on rooms, terrains and a city block of
-// 11k-2M triangles at 256-65,536 probes; with an indirect term the eight
-// corner visibility rays per hit serialise inside a fused lane, and the
-// wavefront kernels are faster.
-uint32_t default_variant(bool indirect)
+// 11k-2M triangles at 256-65,536 probes, and with an indirect term too (its
+// eight corner visibility rays per hit: 411k probes/s against 388k for the
+// wavefront kernels, variant 20, on the 2048-probe room).  Kept per term so
+// that the two can differ again.
+uint32_t default_variant(bool /*indirect*/)
 {
-    return indirect ? kWaveDefault : kFusedDefault;
+    return kFusedDefault;
 }
 constexpr uint32_t kMaxVariant = 24;
 constexpr int kOccTab[3] = {0, 6, 8};
