@@ -1,28 +1,42 @@
 #!/usr/bin/env python3
 """Benchmark of the probe-baking hot path on one MI355X.
 
-Default workload (the plain single-scene bake): procedural room 10x4x10 m
-(64 boxes + 8 icospheres, 11,018 triangles, seed 2), 16x8x16 = 2048 probes,
-4096 Fibonacci rays per probe, one directional light + procedural sky, one
-shadow ray per lit hit, L2 SH (9 coefficients x RGB), FP32 projection.
+Default workload (the batched form, the configuration the metric and its
+target are stated on): 64 bake jobs over four resident scenes (procedural
+rooms of 11k and 99k triangles, fBm terrains of 0.5M and 1.15M), probes per
+job log-uniform in [64, 32768] in a random region, rays per probe from
+{1024, 4096, 16384} (Fibonacci), one directional light + procedural sky,
+one shadow ray per lit hit, L2 SH, BF16 MFMA projection (split operands,
+FP32 accumulation, FP64 merge).  All jobs run as one batch: one persistent
+launch per kernel and round, the rounds captured as one hipGraph, shapes
+picked by the batch autotuner.  A step is one execute of the whole batch
+with every input resident in HBM; `value` is probes/s over the K timed
+steps (device sync on both sides).
 
-A step is one full bake of that grid with every input resident in HBM
-(vlb_plan_execute): the trace stage (closest-hit, shadow and shading kernels
-of the autotuned trace variant), the MFMA projection and the FP64 merge.
-`value` is probes/s over the K timed steps (device sync on both sides).  The
-roofline figure is for the trace stage, timed with hipEvents on the
-library's own stream inside the timed region.
+The same run also reports: the plain FP32 GPU baker on the same 64 jobs
+(one vlb_bake per job, timed warm over whole passes) and the ratio; the
+served form at a fixed Poisson arrival rate (a ~2M-triangle city block
+joins the four scenes; each request moves the sun or adds a point light and
+rebakes 256-65536 probes at 4096 rays) with its p50/p99 latency against the
+plain per-request baker; SH error against the FP64 oracle on seeded probes
+of every job; primary-hit parity against the oracle at the workload's
+sizes; the trace kernel's roofline; and the CPU baseline (the FP32 C
+oracle built -march=native on this host, all usable cores).
 
-    python bench.py [--steps K] [--warmup W] [--precision fp32|bf16|fp16]
-                    [--config c2|c1|c3|c4|c5|copy|sweep|bounce|reference] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W]
+                    [--config c4|c2|c1|c3|c5|copy|sweep|bounce|reference]
+                    [--precision ...] [--variant V] [--no-cpu-baseline]
 
 --no-cpu-baseline skips the CPU leg (the oracle's baseline timing and the
-SH error check against the FP64 oracle).
+SH error / hit checks against the oracle).
 """
 import argparse
 import json
+import math
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -38,6 +52,8 @@ def node_bytes(variant):
     1..12) read float nodes (7 x 16 B in use), the wavefront kernels
     (13..24) quantized ones (4 x 16 B)."""
     return 112.0 if variant <= 12 else 64.0
+
+
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16_fast": 3, "fp16_fast": 4}
 SCHED = {"srpt": 0, "share": 1, "fifo": 2}
 
@@ -60,21 +76,436 @@ def load_traffic(name):
         return None
 
 
+def cpu_threads():
+    """Threads of the CPU baseline: every core this process may run on (its
+    affinity mask), capped by a cgroup CPU quota when one is set; and what
+    the host reports, for the record."""
+    n = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(math.ceil(int(q) / int(per))))
+    except (OSError, ValueError):
+        pass
+    used = min(n, quota) if quota else n
+    return used, {"nproc": os.cpu_count(), "affinity": n, "cgroup_quota": quota}
+
+
+CPU_THREADS, CPU_INFO = cpu_threads()
+_ORC = {}
+
+
 def oracle_leg(args):
     """The CPU leg of a run: the CPU oracle (tests/orc.py over oracle/),
     used for the CPU baseline timing and, as the checker, for the SH error
-    against FP64 -- never for what is measured on the GPU.  None with
-    --no-cpu-baseline, which skips both.  This is the only place bench.py
-    loads the oracle."""
+    and hit parity against FP64 / FP32 -- never for what is measured on the
+    GPU.  None with --no-cpu-baseline, which skips both.  The oracle's
+    sources are built here for this host first (-O3 -march=native
+    -ffp-contract=off, results unchanged); if that fails the portable build
+    is used and the line says so.  This is the only place bench.py loads the
+    oracle."""
     if args.no_cpu_baseline:
         return None
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import orc
+    if "orc" not in _ORC:
+        tmp = tempfile.mkdtemp(prefix="vlb_orc_native_")
+        r = subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "native",
+                            f"NATIVE_DIR={tmp}"], capture_output=True, text=True)
+        lib = os.path.join(tmp, "liborc.so")
+        if r.returncode == 0 and os.path.exists(lib):
+            os.environ["VLB_ORC_LIB"] = lib
+            _ORC["build"] = "gcc -O3 -march=native -ffp-contract=off, built on this host"
+        else:
+            _ORC["build"] = "gcc -O3 -march=x86-64-v3 (native build failed: " + r.stderr[-200:] + ")"
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import orc
 
-    return orc
+        _ORC["orc"] = orc
+    return _ORC["orc"]
 
 
+def cpu_line(value, unit, sample):
+    return {"value": value, "unit": unit, "cores": CPU_THREADS, "kind": "port", "sample": sample,
+            "host": dict(CPU_INFO), "build": _ORC.get("build")}
+
+
+def trace_roofline(kernel, variant, cnt_nodes, cnt_tris, launch_s, launches, profile):
+    """Roofline of a trace stage.  `achieved` is the HBM bytes per launch
+    measured by rocprofv3 (FETCH_SIZE x2 + WRITE_SIZE, committed under
+    profiles/ for this kernel and workload) over the live event-timed launch
+    duration, so frac <= 1 is a real HBM fraction.  `alg_rate_gbs` is the
+    algorithmic node + triangle byte rate (112 or 64 B per BVH4 node visit +
+    48 B per triangle test, counted by the kernels' counting pass): those
+    bytes are mostly served by L1/L2/MALL, so it can exceed the HBM peak."""
+    alg = (node_bytes(variant) * cnt_nodes + TRI_BYTES * cnt_tris) / launches
+    prof = None
+    if profile:
+        prof = profile.get("variants", {}).get(str(variant))
+    traffic = prof["hbm_bytes_per_launch"] if prof else None
+    achieved = traffic / launch_s / 1e9 if traffic else None
+    return {"bound": "hbm", "achieved": None if achieved is None else round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic, "kernel": kernel, "launch_ms": round(launch_s * 1e3, 4),
+            "launches_per_step": launches, "alg_bytes_per_launch": round(alg, 1),
+            "alg_rate_gbs": round(alg / launch_s / 1e9, 1),
+            "profile": (profile or {}).get("file") if prof else None}
+
+
+def _cpu_seconds_per_probe(vlb, oscene, prm, nprobe, min_seconds, seed=0):
+    """Wall seconds per probe of the FP32 CPU oracle (CPU_THREADS threads) on
+    a seeded subset of the job's probes (per-probe cost is uniform in the
+    ray count), baked repeatedly until min_seconds have passed; and the
+    number of probes baked."""
+    import numpy as np
+
+    P = prm.probes.shape[0]
+    rng = np.random.default_rng(seed)
+    sub = np.sort(rng.choice(P, size=min(nprobe, P), replace=False))
+    q = vlb.Params(**{**prm.__dict__, "probes": prm.probes[sub].copy(), "_keep": []})
+    q.precision = 0
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oscene.bake(q, mode64=False, nthreads=CPU_THREADS)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= min_seconds:
+            break
+    return el / (reps * sub.size), int(reps * sub.size)
+
+
+def _err_sample(vlb, oracle_of, items, nprobe=8, seed=0):
+    """Max and p99 rel. SH error vs the FP64 oracle over nprobe seeded probes
+    of each (scene index, params, coefficients) item."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    errs = []
+    for si, prm, got in items:
+        P = prm.probes.shape[0]
+        sub = np.sort(rng.choice(P, size=min(nprobe, P), replace=False))
+        q = vlb.Params(**{**prm.__dict__, "probes": prm.probes[sub].copy(), "_keep": []})
+        q.precision = 0
+        ref = oracle_of(si).bake(q, mode64=True, nthreads=CPU_THREADS)["coeffs"]
+        errs.append(rel_err(got[sub], ref).ravel())
+    allv = np.concatenate(errs)
+    return {"max": float(allv.max()), "p99": float(np.percentile(allv, 99)),
+            "probes": int(allv.size // 3), "items": len(items)}
+
+
+def _hit_parity(ctx, vlb, host, sids, oracle_of, picks, variant):
+    """Primary-hit parity at the workload's sizes: for each (scene index,
+    params) pick, the product's closest-hit prim id per ray (vlb_bake_radiance,
+    trace variant `variant`) against the FP32 oracle's, over every ray of the
+    pick's probes.  Grazing rays (|d.n_g| < 1e-4, or a barycentric < 1e-6 at
+    the oracle's hit) are counted separately."""
+    import numpy as np
+
+    tot = same = graze = graze_same = 0
+    for si, prm in picks:
+        p = vlb.Params(**{**prm.__dict__, "_keep": []})
+        p.precision = 0
+        p.trace_variant = variant
+        _, ghit = ctx.bake_radiance(sids[si], p, want_radiance=False, want_hits=True)
+        osc = oracle_of(si)
+        chit = osc.bake(p, mode64=False, nthreads=CPU_THREADS, want_hits=True)["hits"]
+        dirs, _ = vlb.directions(vlb.DIRS_FIBONACCI, p.rays)
+        P, R = chit.shape
+        org = np.repeat(p.probes, R, axis=0)
+        d = np.tile(dirs, (P, 1))
+        prim, t, u, v = osc.trace_closest(org, d, p.t_min, p.t_max)
+        assert np.array_equal(prim, chit.ravel())
+        xyz, idx = host[si]["xyz"].astype(np.float64), host[si]["idx"]
+        h = prim >= 0
+        tri = idx[prim[h]]
+        n = np.cross(xyz[tri[:, 1]] - xyz[tri[:, 0]], xyz[tri[:, 2]] - xyz[tri[:, 0]])
+        n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-300)
+        cosang = np.abs((n * d[h]).sum(axis=1))
+        bary = np.minimum(np.minimum(u[h], v[h]), 1.0 - u[h] - v[h])
+        gz = np.zeros(prim.shape, bool)
+        gz[np.flatnonzero(h)] = (cosang < 1e-4) | (bary < 1e-6)
+        eq = ghit.ravel() == chit.ravel()
+        tot += eq.size
+        same += int(eq.sum())
+        graze += int(gz.sum())
+        graze_same += int(eq[gz].sum())
+    ng = tot - graze
+    return {"rays": tot, "equal": same, "frac": round(same / tot, 8),
+            "non_grazing_frac": round((same - graze_same) / ng, 8) if ng else None,
+            "grazing": graze, "grazing_equal": graze_same, "variant": variant}
+
+
+# --------------------------------------------------------- the served form
+def serve(ctx, vlb, W, host, sids, args, oracle_of=None):
+    """The served form on resident scenes: args.requests Poisson arrivals at
+    args.rate per second through the server (BF16, continuous batching,
+    HBM-budget admission), then the same schedule through the plain FP32
+    baker, one vlb_bake per request in arrival order.  Returns the result
+    dict and (request params, coefficients) for error checks."""
+    import threading
+
+    import numpy as np
+
+    n = args.requests
+    arrive, reqs, sun0 = W.c5_requests(host, n, args.rate)
+    srv = vlb.Server(ctx, precision=PREC["bf16"], state_budget=args.state_budget << 20,
+                     step_rays=args.step_rays, policy=SCHED[args.policy], trace_variant=args.variant)
+    # warm-up: one small request per scene (direction tables, graph capture)
+    for si in range(len(host)):
+        w = (si, reqs[0][1][:256], sun0,
+             [(vlb.LIGHT_DIRECTIONAL, (-sun0).tolist(), (3.0, 3.0, 3.0))])
+        srv.wait(srv.submit(sids[si], W.c5_params(w, PREC["bf16"])))
+    ps16 = [W.c5_params(r, PREC["bf16"]) for r in reqs]
+    ids = [None] * n
+
+    def feeder():
+        t0 = time.perf_counter()
+        for i in range(n):
+            dt = t0 + arrive[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids[i] = srv.submit(sids[reqs[i][0]], ps16[i])
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    lat, outs, infos_r = [], [], []
+    for i in range(n):
+        while ids[i] is None:
+            time.sleep(0.0005)
+        c, inf = srv.wait(ids[i])
+        outs.append(c)
+        infos_r.append(inf)
+        lat.append(inf["done_ms"] - inf["submit_ms"])
+    th.join()
+    st = srv.stats()
+    first = min(i["submit_ms"] for i in infos_r)
+    last = max(i["done_ms"] for i in infos_r)
+    P_all = sum(r[1].shape[0] for r in reqs)
+    srv_span = (last - first) / 1e3
+    srv.destroy()
+
+    ps32 = [W.c5_params(r, PREC["fp32"]) for r in reqs]
+    plat = []
+    q0 = time.perf_counter()
+    for i in range(n):
+        now = time.perf_counter() - q0
+        if now < arrive[i]:
+            time.sleep(arrive[i] - now)
+        ctx.bake(sids[reqs[i][0]], ps32[i])
+        plat.append((time.perf_counter() - q0 - arrive[i]) * 1e3)
+    plain_span = time.perf_counter() - q0
+    lat = np.asarray(lat)
+    plat = np.asarray(plat)
+    res = {"requests": n, "rate_per_s": args.rate, "probes": P_all, "rays_per_probe": W.C5_RAYS,
+           "probes_per_s": round(P_all / srv_span, 1),
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                          "p99": round(float(np.percentile(lat, 99)), 2),
+                          "max": round(float(lat.max()), 2)},
+           "server": {"steps": st["steps"], "busy_ms": round(st["busy_ms"], 1),
+                      "active_peak": st["active_peak"], "state_bytes_peak": st["state_bytes_peak"],
+                      "span_s": round(srv_span, 3), "policy": args.policy,
+                      "state_budget_mib": args.state_budget, "step_rays": args.step_rays,
+                      "trace_variant": args.variant or 12},
+           "plain_fp32_per_request": {"probes_per_s": round(P_all / plain_span, 1),
+                                      "p50_ms": round(float(np.percentile(plat, 50)), 2),
+                                      "p99_ms": round(float(np.percentile(plat, 99)), 2),
+                                      "span_s": round(plain_span, 3)}}
+    return res, reqs, ps16, ps32, outs
+
+
+# ----------------------------------------------------------- batched form
+def bench_c4(args):
+    """The default line: the batched form (see the module docstring)."""
+    import numpy as np
+
+    import vlb
+    import workloads as W
+
+    ctx = vlb.Context(0)
+    host = W.scenes(W.C5_SCENES)          # the city block first, then the batched form's four
+    sids = [ctx.upload_scene_dict(s) for s in host]
+    infos = [ctx.scene_info(s) for s in sids]
+    spec, sun = W.c4_jobs(host[1:])       # scene indices into host[1:]
+    jobs16 = [(sids[1 + si], W.sun_params(pr, R, PREC["bf16"], sun)) for si, pr, R in spec]
+    P_all = sum(pr.shape[0] for _, pr, _ in spec)
+    rays_all = sum(pr.shape[0] * R for _, pr, R in spec)
+
+    if args.variant:
+        b = ctx.batch(jobs16, trace_variant=args.variant)
+        variant, chunk = args.variant, 256
+    else:
+        b = ctx.batch(jobs16)
+        variant, chunk = b.autotune(2)
+    b.count()
+    cnt = b.info()
+    b.use_graph(True)
+    b.set_timing(True)
+    for _ in range(max(1, args.warmup)):
+        b.execute()
+    b.sync()
+    b.set_timing(True)                    # the means cover the timed executes only
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.execute()
+    b.sync()
+    t1 = time.perf_counter()
+    info = b.info()
+    el = (t1 - t0) / args.steps
+    got = [b.download(j) for j in range(len(jobs16))]
+    b.destroy()
+
+    # the plain FP32 GPU baker: one synchronous vlb_bake per job, in order;
+    # one untimed pass, then whole passes timed
+    jobs32 = [(sids[1 + si], W.sun_params(pr, R, PREC["fp32"], sun)) for si, pr, R in spec]
+    for sid, prm in jobs32:
+        ctx.bake(sid, prm)
+    passes = max(1, args.plain_passes)
+    p0 = time.perf_counter()
+    for _ in range(passes):
+        for sid, prm in jobs32:
+            ctx.bake(sid, prm)
+    plain_s = (time.perf_counter() - p0) / passes
+
+    # the served form, same context and resident scenes
+    served = None
+    if args.requests > 0:
+        served, *_ = serve(ctx, vlb, W, host, sids, args)
+
+    orc = oracle_leg(args)
+    err = hits = cpu = None
+    if orc is not None:
+        oscenes = {}
+
+        def oracle_of(i):          # index into host
+            if i not in oscenes:
+                oscenes[i] = orc.Scene.from_dict(host[i])
+            return oscenes[i]
+
+        err = _err_sample(vlb, oracle_of, [(1 + spec[j][0], jobs16[j][1], got[j]) for j in range(len(spec))],
+                          nprobe=3)
+        # hit parity: 2 seeded probes of the first job on each scene
+        rng = np.random.default_rng(8)
+        picks, seen = [], set()
+        for j, (si, pr, R) in enumerate(spec):
+            if si in seen:
+                continue
+            seen.add(si)
+            sub = np.sort(rng.choice(pr.shape[0], size=2, replace=False))
+            picks.append((1 + si, W.sun_params(pr[sub].copy(), R, PREC["fp32"], sun)))
+        hits = _hit_parity(ctx, vlb, host, sids, oracle_of, picks, variant)
+        # CPU baseline: every job on the FP32 oracle, 16 seeded probes each,
+        # scaled to the job's probe count (per-probe cost is uniform in a job)
+        est, t_cpu = 0.0, 0.0
+        for j, (si, pr, R) in enumerate(spec):
+            spp, n = _cpu_seconds_per_probe(vlb, oracle_of(1 + si), jobs32[j][1], 16, args.cpu_seconds / 64,
+                                            seed=j)
+            est += spp * pr.shape[0]
+            t_cpu += spp * n
+        cpu = cpu_line(round(P_all / est, 2), "probes/s",
+                       f"FP32 C oracle, the same 64 jobs one after another, 16 seeded probes of each baked "
+                       f"repeatedly for >= {args.cpu_seconds / 64:.2f} s and scaled to the job's probe count, "
+                       f"{CPU_THREADS} threads, {t_cpu:.1f} s")
+    ctx.close()
+
+    rays = cnt["primary_rays"] + cnt["shadow_rays"]
+    rounds = info["rounds"]
+    roof = trace_roofline("k_fused_jobs" if variant <= 12 else "k_rays_jobs + k_shade_jobs", variant,
+                          cnt["node_visits"], cnt["tri_tests"], info["trace_ms"] / 1e3 / rounds, rounds,
+                          load_traffic("batch_c4_traffic.json"))
+    return {
+        "metric": METRIC, "value": round(P_all / el, 1), "unit": "probes/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": max(1, args.warmup), "ms_per_step": round(el * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic (4 seeded rooms/terrains, 64 seeded jobs, Fibonacci rays; city block added "
+                "for the served form)",
+        "config": {"workload": "c4 batched form", "jobs": len(spec), "probes": P_all,
+                   "primary_rays": rays_all,
+                   "scenes": [{"scene": W.C4_SCENES[i][0], "triangles": infos[1 + i]["num_tris"],
+                               "bvh_depth": infos[1 + i]["depth"]} for i in range(4)],
+                   "rays_per_probe": list(W.C4_RAYS), "sh_order": 2,
+                   "precision": "bf16 (split operands, FP32 accumulate, FP64 merge)",
+                   "rounds": rounds, "trace_variant": variant, "chunk": chunk, "graph": True},
+        "grays_per_s": round(rays / el / 1e9, 3),
+        "probe_rays_per_s": round(rays_all / el, 1),
+        "shadow_rays_per_step": cnt["shadow_rays"],
+        "per_ray": {"inner_nodes": round(cnt["node_visits"] / rays, 2),
+                    "triangles": round(cnt["tri_tests"] / rays, 2)},
+        "kernel_ms": {"trace": round(info["trace_ms"], 3), "project": round(info["project_ms"], 3),
+                      "merge": round(info["merge_ms"], 3), "timed_runs": info["kernel_timed_runs"],
+                      "device_total": round(info["total_ms"], 3)},
+        "sh_rel_err_vs_fp64": err,
+        "hit_parity": hits,
+        "roofline": roof,
+        "plain_fp32_baker": {"probes_per_s": round(P_all / plain_s, 1), "seconds_per_pass": round(plain_s, 4),
+                             "passes": passes, "calls_per_pass": len(jobs32)},
+        "speedup_vs_plain_fp32": round(plain_s / el, 3),
+        "served": served,
+        "cpu_baseline": cpu,
+        "device_bytes": info["device_bytes"],
+    }
+
+
+def bench_c5(args):
+    """The served form on its own line: the serving leg of the default run,
+    with the SH error of the served requests and the CPU baseline on the
+    request mix."""
+    import numpy as np
+
+    import vlb
+    import workloads as W
+
+    ctx = vlb.Context(0)
+    host = W.scenes(W.C5_SCENES)
+    sids = [ctx.upload_scene_dict(s) for s in host]
+    infos = [ctx.scene_info(s) for s in sids]
+    res, reqs, ps16, ps32, outs = serve(ctx, vlb, W, host, sids, args)
+    ctx.close()
+    orc = oracle_leg(args)
+    err = cpu = None
+    if orc is not None:
+        oscenes = {}
+
+        def oracle_of(i):
+            if i not in oscenes:
+                oscenes[i] = orc.Scene.from_dict(host[i])
+            return oscenes[i]
+
+        n = len(reqs)
+        err = _err_sample(vlb, oracle_of, [(reqs[i][0], ps16[i], outs[i]) for i in range(0, n, max(1, n // 20))])
+        spp_scene, t_cpu = {}, 0.0
+        for i in range(n):
+            si = reqs[i][0]
+            if si in spp_scene:
+                continue
+            spp, m = _cpu_seconds_per_probe(vlb, oracle_of(si), ps32[i], 16, args.cpu_seconds / len(host), seed=i)
+            spp_scene[si] = spp
+            t_cpu += spp * m
+        est = sum(spp_scene[r[0]] * r[1].shape[0] for r in reqs)
+        cpu = cpu_line(round(res["probes"] / est, 2), "probes/s",
+                       f"FP32 C oracle, 16 probes of one request per scene ({len(spp_scene)} scenes) baked "
+                       f"repeatedly and scaled to all {n} requests, {CPU_THREADS} threads, {t_cpu:.1f} s")
+    lat = res["latency_ms"]
+    return {"metric": METRIC, "value": res["probes_per_s"], "unit": "probes/s", "n_gpus": 1,
+            "steps": res["requests"], "warmup": len(host), "ms_per_step": lat["p50"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "data": "synthetic (seeded city + 4 rooms/terrains, Poisson arrivals, Fibonacci rays)",
+            "config": {"workload": "c5 serving form", "requests": res["requests"], "rate_per_s": args.rate,
+                       "probes": res["probes"], "rays_per_probe": W.C5_RAYS,
+                       "scenes": [{"scene": W.C5_SCENES[i][0], "triangles": infos[i]["num_tris"],
+                                   "bvh_depth": infos[i]["depth"]} for i in range(len(host))]},
+            "latency_ms": lat, "server": res["server"],
+            "plain_fp32_per_request": res["plain_fp32_per_request"],
+            "sh_rel_err_vs_fp64": err, "cpu_baseline": cpu}
+
+
+# -------------------------------------------------------- plain single bake
 def bench_c2(args):
+    """The plain single-scene bake: procedural room 10x4x10 m (64 boxes + 8
+    icospheres, 11,018 triangles, seed 2), 16x8x16 = 2048 probes, 4096
+    Fibonacci rays per probe, one directional light + procedural sky, L2 SH,
+    FP32 projection.  A step is one full vlb_plan_execute (trace stage, MFMA
+    projection, FP64 merge)."""
     import numpy as np
 
     import vlb
@@ -107,24 +538,12 @@ def bench_c2(args):
     st = plan.stats()
     coeffs = plan.download()
     elapsed = t1 - t0
-    ms_per_step = 1e3 * elapsed / args.steps
     probes_per_s = P * args.steps / elapsed
     rays_per_step = cnt["primary_rays"] + cnt["shadow_rays"]
     grays = rays_per_step * args.steps / elapsed / 1e9
+    roof = trace_roofline("k_trace" if variant <= 12 else "k_rays + k_shade", variant, cnt["node_visits"],
+                          cnt["tri_tests"], st["trace_ms"] / 1e3, 1, load_traffic("bake_c2_traffic.json"))
 
-    # algorithmic bytes of one trace stage: the node bytes of the variant
-    # per BVH4 node fetched + 48 B per triangle tested (the 48 B shading record per primary hit is
-    # not counted), both counted by the kernels' counting variant
-    alg_bytes = node_bytes(variant) * cnt["node_visits"] + TRI_BYTES * cnt["tri_tests"]
-    trace_s = st["trace_ms"] / 1e3
-    achieved = alg_bytes / trace_s / 1e9
-    traffic = load_traffic("bake_c2_traffic.json")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None}
-
-    # CPU leg: error of this run against the FP64 oracle on a seeded probe
-    # subset, and the CPU baseline
     orc = oracle_leg(args)
     err_line, cpu = None, None
     if orc is not None:
@@ -133,26 +552,23 @@ def bench_c2(args):
         psub = vlb.config_c2_params(scene, seed=2)
         psub.probes = params.probes[sub].copy()
         oscene = orc.Scene.from_dict(scene)
-        ref = oscene.bake(psub, mode64=True, nthreads=min(16, os.cpu_count() or 1))["coeffs"]
+        ref = oscene.bake(psub, mode64=True, nthreads=CPU_THREADS)["coeffs"]
         err = rel_err(coeffs[sub], ref)
         err_line = {"max": float(err.max()), "p99": float(np.percentile(err, 99)), "probes": len(sub)}
         # the FP32 C oracle on the same scene, probes and rays: whole c2
-        # bakes repeated until >= 10 s of wall time on the host's cores
-        nthreads = min(16, os.cpu_count() or 1)
+        # bakes repeated until >= cpu_seconds of wall time on the host's cores
         pc = vlb.config_c2_params(scene, seed=2)
         reps = 0
         c0 = time.perf_counter()
         while True:
-            oscene.bake(pc, mode64=False, nthreads=nthreads)
+            oscene.bake(pc, mode64=False, nthreads=CPU_THREADS)
             reps += 1
-            if time.perf_counter() - c0 >= 10.0:
+            if time.perf_counter() - c0 >= args.cpu_seconds:
                 break
         c1 = time.perf_counter()
-        cpu = {"value": round(reps * P / (c1 - c0), 2), "unit": "probes/s", "cores": nthreads,
-               "kind": "port",
-               "sample": f"FP32 C oracle: {reps} full bakes of the same {P} probes x {R} rays "
-                         f"(same scene, BVH-independent results), {nthreads} threads, {c1 - c0:.1f} s"}
-
+        cpu = cpu_line(round(reps * P / (c1 - c0), 2), "probes/s",
+                       f"FP32 C oracle: {reps} full bakes of the same {P} probes x {R} rays "
+                       f"(same scene, BVH-independent results), {CPU_THREADS} threads, {c1 - c0:.1f} s")
     line = {
         "metric": METRIC,
         "value": round(probes_per_s, 2),
@@ -160,7 +576,7 @@ def bench_c2(args):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -174,13 +590,12 @@ def bench_c2(args):
         "probe_rays_per_s": round(P * R * args.steps / elapsed, 1),
         "shadow_rays_per_step": cnt["shadow_rays"],
         "per_ray": {"inner_nodes": round(cnt["node_visits"] / rays_per_step, 2),
-                    "triangles": round(cnt["tri_tests"] / rays_per_step, 2),
-                    "alg_bytes": round(alg_bytes / rays_per_step, 1)},
+                    "triangles": round(cnt["tri_tests"] / rays_per_step, 2)},
         "trace_variant": variant,
         "kernel_ms": {"trace": round(st["trace_ms"], 4), "project": round(st["project_ms"], 4),
                       "merge": round(st["merge_ms"], 4), "timed_runs": st["timed_runs"]},
         "sh_rel_err_vs_fp64": err_line,
-        "roofline": roofline,
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     plan.destroy()
@@ -233,15 +648,29 @@ def bench_c1(args):
     err, cpu = None, None
     if orc is not None:
         err = rel_err(out[0], orc.skybox_sh(base[0], 9)).max()
-        # CPU leg: the oracle's projection (one thread, FP64) of the same cubemaps
-        n_cpu, c0 = 0, time.perf_counter()
-        while n_cpu < 8 or time.perf_counter() - c0 < 5.0:
-            orc.skybox_sh(base[n_cpu % 8], 9)
-            n_cpu += 1
+        # CPU leg: the oracle's projection (FP64) of the same cubemaps, one
+        # cubemap per thread at a time on every usable core (the ctypes call
+        # releases the GIL)
+        import threading
+
+        counts = [0] * CPU_THREADS
+        c0 = time.perf_counter()
+
+        def worker(k):
+            while time.perf_counter() - c0 < args.cpu_seconds / 2:
+                orc.skybox_sh(base[(counts[k] + k) % 8], 9)
+                counts[k] += 1
+
+        ths = [threading.Thread(target=worker, args=(k,)) for k in range(CPU_THREADS)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
         c1 = time.perf_counter()
-        cpu = {"value": round(n_cpu / (c1 - c0), 1), "unit": "skyboxes/s", "cores": 1, "kind": "port",
-               "sample": f"C oracle (FP64, exact solid angles), {n_cpu} of the same 6x128x128 cubemaps "
-                         f"projected one after another on one thread, {c1 - c0:.1f} s"}
+        n_cpu = sum(counts)
+        cpu = cpu_line(round(n_cpu / (c1 - c0), 1), "skyboxes/s",
+                       f"C oracle (FP64, exact solid angles), {n_cpu} of the same 6x128x128 cubemaps, "
+                       f"{CPU_THREADS} threads each projecting whole cubemaps, {c1 - c0:.1f} s")
     kms = float(np.mean(proj))
     traffic = load_traffic("skybox_c1_traffic.json")
     achieved = (nbytes + B * 27 * 4) / (kms / 1e3) / 1e9
@@ -345,15 +774,15 @@ def bench_c3(args):
     cpu = None
     if osc is not None:
         # CPU leg: the FP32 oracle on a fixed seeded subset of 256 probes, L3
-        spp, n = _cpu_seconds_per_probe(vlb, osc, params(0, 3), 256, 5.0, seed=3)
-        cpu = {"value": round(1.0 / spp, 2), "unit": "probes/s", "cores": CPU_THREADS, "kind": "port",
-               "sample": f"FP32 C oracle, a fixed seeded subset of 256 of the 16,384 probes baked repeatedly "
-                         f"({n} probe bakes of 16,384 rays, L3, same scene, BVH-independent results), "
-                         f"{CPU_THREADS} threads, {spp * n:.1f} s"}
+        spp, n = _cpu_seconds_per_probe(vlb, osc, params(0, 3), 256, args.cpu_seconds / 2, seed=3)
+        cpu = cpu_line(round(1.0 / spp, 2), "probes/s",
+                       f"FP32 C oracle, a fixed seeded subset of 256 of the 16,384 probes baked repeatedly "
+                       f"({n} probe bakes of 16,384 rays, L3, same scene, BVH-independent results), "
+                       f"{CPU_THREADS} threads, {spp * n:.1f} s")
     errs = {}
     for order in ((2, 3) if osc is not None else ()):
         ref = osc.bake(params(0, order, probes[sub].copy()), mode64=True,
-                       nthreads=min(16, os.cpu_count() or 1))["coeffs"]
+                       nthreads=CPU_THREADS)["coeffs"]
         K = 9 if order == 2 else 16
         for name, prec in (("fp32", 0), ("bf16", 1), ("fp16", 2), ("bf16_fast", 3), ("fp16_fast", 4)):
             got = ctx.bake(sid, params(prec, order, probes[sub].copy()))
@@ -364,7 +793,6 @@ def bench_c3(args):
             band["all"] = float(rel_err(got, ref).max())
             errs[f"L{order}_{name}"] = band
     rays = cnt["primary_rays"] + cnt["shadow_rays"]
-    alg = node_bytes(variant) * cnt["node_visits"] + TRI_BYTES * cnt["tri_tests"]
     tr = runs["bf16"]["trace_ms"] / 1e3
     line = {"metric": METRIC, "value": runs["bf16"]["probes_per_s"], "unit": "probes/s", "n_gpus": 1,
             "steps": args.steps, "warmup": 1, "ms_per_step": runs["bf16"]["ms_per_step"],
@@ -378,341 +806,9 @@ def bench_c3(args):
                         "triangles": round(cnt["tri_tests"] / rays, 2)},
             "sh_rel_err_vs_fp64_by_band": errs or None, "error_probes": len(sub) if errs else 0,
             "cpu_baseline": cpu,
-            "roofline": {"bound": "hbm", "achieved": round(alg / tr / 1e9, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(alg / tr / 1e9 / HBM_PEAK_GBS, 4), "traffic": None}}
-    ctx.close()
-    return line
-
-
-# ------------------------------------------------------------ c4 / c5 data
-C4_SCENES = (  # (label, kind, seed, param0, param1): ~10k, ~100k, ~500k, ~1.1M triangles
-    ("room 11k", 0, 40, 0, 0),
-    ("room 99k", 0, 41, 64, 77),
-    ("terrain 499k", 1, 42, 256, 18),
-    ("terrain 1.15M", 1, 43, 256, 50),
-)
-
-
-def _region_probes(rng, scene, n):
-    """n probes uniformly in a random axis-aligned region of the scene bounds."""
-    import numpy as np
-
-    lo = np.asarray(scene["bounds_min"], np.float64)
-    hi = np.asarray(scene["bounds_max"], np.float64)
-    ext = hi - lo
-    size = ext * rng.uniform(0.25, 1.0, 3)
-    start = lo + rng.uniform(0.0, 1.0, 3) * (ext - size)
-    return (start + rng.uniform(0.0, 1.0, (n, 3)) * size).astype(np.float32)
-
-
-def _sun_params(vlb, probes, rays, prec, sun, lights=None):
-    return vlb.Params(probes=probes, rays=rays, sh_order=2, precision=prec,
-                      shade_mode=vlb.SHADE_LAMBERT, sun_dir=sun.tolist(),
-                      lights=lights if lights is not None else
-                      [(vlb.LIGHT_DIRECTIONAL, (-sun).tolist(), (3.0, 3.0, 3.0))],
-                      shadow_bias=2e-3)
-
-
-CPU_THREADS = min(16, os.cpu_count() or 1)
-
-
-def _cpu_seconds_per_probe(vlb, oscene, prm, nprobe, min_seconds, seed=0):
-    """Wall seconds per probe of the FP32 CPU oracle (CPU_THREADS threads) on
-    a seeded subset of the job's probes (per-probe cost is uniform in the
-    ray count), baked repeatedly until min_seconds have passed; and the
-    number of probes baked."""
-    import numpy as np
-
-    P = prm.probes.shape[0]
-    rng = np.random.default_rng(seed)
-    sub = np.sort(rng.choice(P, size=min(nprobe, P), replace=False))
-    q = vlb.Params(**{**prm.__dict__, "probes": prm.probes[sub].copy(), "_keep": []})
-    q.precision = 0
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        oscene.bake(q, mode64=False, nthreads=CPU_THREADS)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= min_seconds:
-            break
-    return el / (reps * sub.size), int(reps * sub.size)
-
-
-def _err_sample(vlb, oscenes, items, nprobe=8, seed=0):
-    """Max rel. SH error vs the FP64 oracle over nprobe probes of each
-    (scene index, params, coefficients) item."""
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    worst = 0.0
-    errs = []
-    for si, prm, got in items:
-        P = prm.probes.shape[0]
-        sub = np.sort(rng.choice(P, size=min(nprobe, P), replace=False))
-        q = vlb.Params(**{**prm.__dict__, "probes": prm.probes[sub].copy(), "_keep": []})
-        q.precision = 0
-        ref = oscenes[si].bake(q, mode64=True, nthreads=min(16, os.cpu_count() or 1))["coeffs"]
-        e = rel_err(got[sub], ref)
-        errs.append(e)
-        worst = max(worst, float(e.max()))
-    allv = np.concatenate([e.ravel() for e in errs])
-    return {"max": worst, "p99": float(np.percentile(allv, 99)), "probes": int(allv.size // 3)}
-
-
-def bench_c4(args):
-    """Batched form: 64 uneven jobs over 4 resident scenes (11k .. 1.15M
-    triangles), probes per job log-uniform in [64, 32768], rays per probe in
-    {1024, 4096, 16384}, L2, BF16 projection with FP32 accumulation, all jobs
-    in one batch (one persistent launch per pass per round, hipGraph),
-    shapes chosen by the batch autotuner; against the plain FP32 baker
-    (one vlb_bake call per job)."""
-    import numpy as np
-
-    import vlb
-
-    ctx = vlb.Context(0)
-    scenes, sids, infos = [], [], []
-    for label, kind, seed, p0, p1 in C4_SCENES:
-        sc = vlb.generate_scene(kind, seed, p0, p1)
-        scenes.append(sc)
-        sids.append(ctx.upload_scene_dict(sc))
-        infos.append(ctx.scene_info(sids[-1]))
-    rng = np.random.default_rng(4)
-    sun = vlb.sun_direction(rng)
-    spec = []
-    for j in range(64):
-        si = int(rng.integers(0, 4))
-        P = int(round(np.exp(rng.uniform(np.log(64), np.log(32768)))))
-        R = int(rng.choice([1024, 4096, 16384]))
-        spec.append((si, _region_probes(rng, scenes[si], P), R))
-    jobs16 = [(sids[si], _sun_params(vlb, pr, R, PREC["bf16"], sun)) for si, pr, R in spec]
-    P_all = sum(pr.shape[0] for _, pr, _ in spec)
-    rays_all = sum(pr.shape[0] * R for _, pr, R in spec)
-
-    b = ctx.batch(jobs16)
-    variant, chunk = b.autotune(2)
-    b.count()
-    cnt = b.info()
-    b.use_graph(True)
-    for _ in range(max(1, args.warmup)):
-        b.execute()
-    b.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        b.execute()
-    b.sync()
-    t1 = time.perf_counter()
-    info = b.info()
-    el = (t1 - t0) / args.steps
-    got = [b.download(j) for j in range(len(jobs16))]
-
-    # the plain FP32 baker: one synchronous vlb_bake per job, in order
-    jobs32 = [(sids[si], _sun_params(vlb, pr, R, PREC["fp32"], sun)) for si, pr, R in spec]
-    p0 = time.perf_counter()
-    for sid, prm in jobs32:
-        ctx.bake(sid, prm)
-    p1 = time.perf_counter()
-    plain_s = p1 - p0
-
-    # CPU leg: error vs FP64 on 8 probes of one job per scene
-    orc = oracle_leg(args)
-    err, cpu = None, None
-    if orc is not None:
-        oscenes = [orc.Scene.from_dict(sc) for sc in scenes]
-        pick = {}
-        for j, (si, pr, R) in enumerate(spec):
-            pick.setdefault(si, j)
-        items = [(spec[j][0], jobs16[j][1], got[j]) for j in sorted(pick.values())]
-        err = _err_sample(vlb, oscenes, items)
-        # CPU leg: every job on the FP32 oracle, 16 seeded probes each, scaled
-        # to the job's probe count (per-probe cost is uniform within a job)
-        est, t_cpu = 0.0, 0.0
-        for j, (si, pr, R) in enumerate(spec):
-            spp, n = _cpu_seconds_per_probe(vlb, oscenes[si], jobs32[j][1], 16, 0.15, seed=j)
-            est += spp * pr.shape[0]
-            t_cpu += spp * n
-        cpu = {"value": round(P_all / est, 2), "unit": "probes/s", "cores": CPU_THREADS, "kind": "port",
-               "sample": f"FP32 C oracle, the same 64 jobs serially, 16 seeded probes of each baked repeatedly "
-                         f"for >= 0.15 s and scaled to the job's size, {CPU_THREADS} threads, {t_cpu:.1f} s"}
-
-    rays = cnt["primary_rays"] + cnt["shadow_rays"]
-    line = {"metric": METRIC, "value": round(P_all / el, 1), "unit": "probes/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": max(1, args.warmup), "ms_per_step": round(el * 1e3, 2),
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
-            "data": "synthetic (4 seeded rooms/terrains, 64 seeded jobs, Fibonacci rays)",
-            "config": {"workload": "c4 batched form", "jobs": len(spec), "probes": P_all,
-                       "primary_rays": rays_all,
-                       "scenes": [{"scene": C4_SCENES[i][0], "triangles": infos[i]["num_tris"],
-                                   "bvh_depth": infos[i]["depth"]} for i in range(4)],
-                       "rounds": info["rounds"], "trace_variant": variant, "chunk": chunk},
-            "grays_per_s": round(rays / el / 1e9, 3),
-            "device_ms_per_batch": round(info["total_ms"], 2),
-            "per_ray": {"inner_nodes": round(cnt["node_visits"] / rays, 2),
-                        "triangles": round(cnt["tri_tests"] / rays, 2)},
-            "plain_fp32_baker": {"probes_per_s": round(P_all / plain_s, 1), "seconds": round(plain_s, 3),
-                                 "calls": len(jobs32)},
-            "speedup_vs_plain_fp32": round(plain_s / el, 3),
-            "sh_rel_err_vs_fp64": err,
-            "cpu_baseline": cpu,
-            "device_bytes": info["device_bytes"]}
-    b.destroy()
-    ctx.close()
-    return line
-
-
-def bench_c5(args):
-    """Serving form: a city block (400 buildings, ~2M triangles) and the 4
-    scenes of the batched form stay resident; rebake requests arrive as a
-    Poisson process (--rate per second); each moves the sun or adds a point
-    light (intensity U[10, 500]) and rebakes 256..65536 probes (log-uniform)
-    of a random region at 4096 rays per probe.  The server (BF16, continuous
-    batching, HBM-budget admission) against the plain per-request FP32
-    baker (vlb_bake per request, in arrival order) on the same schedule."""
-    import threading
-
-    import numpy as np
-
-    import vlb
-
-    ctx = vlb.Context(0)
-    specs = (("city 2M", 2, 5, 0, 0),) + C4_SCENES
-    scenes, sids, infos = [], [], []
-    for label, kind, seed, p0, p1 in specs:
-        sc = vlb.generate_scene(kind, seed, p0, p1)
-        scenes.append(sc)
-        sids.append(ctx.upload_scene_dict(sc))
-        infos.append(ctx.scene_info(sids[-1]))
-    rng = np.random.default_rng(5)
-    sun0 = vlb.sun_direction(rng)
-    n = args.requests
-    gaps = rng.exponential(1.0 / args.rate, n)
-    arrive = np.cumsum(gaps) - gaps[0]
-    reqs = []
-    for i in range(n):
-        si = int(rng.integers(0, len(scenes)))
-        P = int(round(np.exp(rng.uniform(np.log(256), np.log(65536)))))
-        pr = _region_probes(rng, scenes[si], P)
-        if rng.uniform() < 0.5:   # the sun moves
-            sun = sun0 + rng.normal(0.0, 0.2, 3)
-            sun[1] = abs(sun[1]) + 0.1
-            sun /= np.linalg.norm(sun)
-            lights = [(vlb.LIGHT_DIRECTIONAL, (-sun).tolist(), (3.0, 3.0, 3.0))]
-        else:                     # a point light is added
-            sun = sun0
-            pos = pr[int(rng.integers(0, P))] + rng.normal(0.0, 1.0, 3).astype(np.float32)
-            lights = [(vlb.LIGHT_DIRECTIONAL, (-sun).tolist(), (3.0, 3.0, 3.0)),
-                      (vlb.LIGHT_POINT, pos.tolist(), (float(rng.uniform(10, 500)),) * 3)]
-        reqs.append((si, pr, sun, lights))
-
-    def params(r, prec):
-        si, pr, sun, lights = r
-        return _sun_params(vlb, pr, 4096, prec, sun, lights)
-
-    # ---- the server
-    srv = vlb.Server(ctx, precision=PREC["bf16"], state_budget=args.state_budget << 20,
-                     step_rays=args.step_rays, policy=SCHED[args.policy],
-                     trace_variant=args.variant)
-    # warm-up: one small request per scene (direction tables, graph capture)
-    for si in range(len(scenes)):
-        w = (si, reqs[0][1][:256], sun0, [(vlb.LIGHT_DIRECTIONAL, (-sun0).tolist(), (3.0, 3.0, 3.0))])
-        srv.wait(srv.submit(sids[si], params(w, PREC["bf16"])))
-    ps16 = [params(r, PREC["bf16"]) for r in reqs]
-    ids = [None] * n
-    base = {}
-
-    def feeder():
-        t0 = time.perf_counter()
-        base["t0"] = t0
-        for i in range(n):
-            dt = t0 + arrive[i] - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
-            ids[i] = srv.submit(sids[reqs[i][0]], ps16[i])
-
-    th = threading.Thread(target=feeder)
-    th.start()
-    lat, outs, infos_r = [], [], []
-    for i in range(n):
-        while ids[i] is None:
-            time.sleep(0.0005)
-        c, inf = srv.wait(ids[i])
-        outs.append(c)
-        infos_r.append(inf)
-        lat.append(inf["done_ms"] - inf["submit_ms"])
-    th.join()
-    st = srv.stats()
-    first = min(i["submit_ms"] for i in infos_r)
-    last = max(i["done_ms"] for i in infos_r)
-    P_all = sum(r[1].shape[0] for r in reqs)
-    srv_span = (last - first) / 1e3
-    srv.destroy()
-
-    # ---- the plain per-request FP32 baker on the same arrival schedule
-    ps32 = [params(r, PREC["fp32"]) for r in reqs]
-    plat = []
-    q0 = time.perf_counter()
-    for i in range(n):
-        now = time.perf_counter() - q0
-        if now < arrive[i]:
-            time.sleep(arrive[i] - now)
-        ctx.bake(sids[reqs[i][0]], ps32[i])
-        plat.append((time.perf_counter() - q0 - arrive[i]) * 1e3)
-    plain_span = time.perf_counter() - q0
-
-    # CPU leg: error vs FP64 on 8 probes of a few requests, and the FP32
-    # oracle's rate on the request mix (16 probes of one request per scene,
-    # scaled to every request of that scene)
-    orc = oracle_leg(args)
-    err, cpu = None, None
-    if orc is not None:
-        oscenes = {}
-        items = []
-        for i in range(0, n, max(1, n // 5)):
-            si = reqs[i][0]
-            if si not in oscenes:
-                oscenes[si] = orc.Scene.from_dict(scenes[si])
-            items.append((si, ps16[i], outs[i]))
-        err = _err_sample(vlb, oscenes, items)
-        spp_scene, t_cpu = {}, 0.0
-        for i in range(n):
-            si = reqs[i][0]
-            if si in spp_scene:
-                continue
-            if si not in oscenes:
-                oscenes[si] = orc.Scene.from_dict(scenes[si])
-            spp, m = _cpu_seconds_per_probe(vlb, oscenes[si], ps32[i], 16, 2.0, seed=i)
-            spp_scene[si] = spp
-            t_cpu += spp * m
-        est = sum(spp_scene[r[0]] * r[1].shape[0] for r in reqs)
-        cpu = {"value": round(P_all / est, 2), "unit": "probes/s", "cores": CPU_THREADS, "kind": "port",
-               "sample": f"FP32 C oracle, 16 probes of one request per scene ({len(spp_scene)} scenes) baked "
-                         f"repeatedly for >= 2 s each and scaled to all {n} requests, {CPU_THREADS} threads, "
-                         f"{t_cpu:.1f} s"}
-
-    lat = np.asarray(lat)
-    plat = np.asarray(plat)
-    line = {"metric": METRIC, "value": round(P_all / srv_span, 1), "unit": "probes/s", "n_gpus": 1,
-            "steps": n, "warmup": len(scenes), "ms_per_step": round(float(np.mean(lat)), 2),
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
-            "data": "synthetic (seeded city + 4 rooms/terrains, Poisson arrivals, Fibonacci rays)",
-            "config": {"workload": "c5 serving form", "requests": n, "rate_per_s": args.rate,
-                       "probes": P_all, "rays_per_probe": 4096,
-                       "scenes": [{"scene": specs[i][0], "triangles": infos[i]["num_tris"],
-                                   "bvh_depth": infos[i]["depth"]} for i in range(len(specs))],
-                       "state_budget_mib": args.state_budget, "step_rays": args.step_rays,
-                       "policy": args.policy, "trace_variant": args.variant or 12},
-            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
-                           "p99": round(float(np.percentile(lat, 99)), 2),
-                           "max": round(float(lat.max()), 2)},
-            "server": {"steps": st["steps"], "busy_ms": round(st["busy_ms"], 1),
-                       "active_peak": st["active_peak"], "state_bytes_peak": st["state_bytes_peak"],
-                       "span_s": round(srv_span, 3)},
-            "plain_fp32_per_request": {"probes_per_s": round(P_all / plain_span, 1),
-                                       "p50_ms": round(float(np.percentile(plat, 50)), 2),
-                                       "p99_ms": round(float(np.percentile(plat, 99)), 2),
-                                       "span_s": round(plain_span, 3)},
-            "sh_rel_err_vs_fp64": err,
-            "cpu_baseline": cpu}
+            "roofline": trace_roofline("k_trace" if variant <= 12 else "k_rays + k_shade", variant,
+                                       cnt["node_visits"], cnt["tri_tests"], tr, 1,
+                                       load_traffic("bake_c3_traffic.json"))}
     ctx.close()
     return line
 
@@ -769,16 +865,16 @@ def bench_reference(args):
     if orc is not None:
         osc = orc.Scene.from_dict(scene)
         sub = np.array([0, P // 2])
-        nth = min(16, os.cpu_count() or 1)
+        nth = CPU_THREADS
         ref = osc.bake(params(probes[sub].copy(), PREC["fp32"]), mode64=True, nthreads=nth)["coeffs"]
         e = rel_err(got[sub], ref)
         err = {"max": float(e.max()), "probes": int(sub.size)}
         c0 = time.perf_counter()
         osc.bake(params(probes[sub].copy(), PREC["fp32"]), mode64=False, nthreads=nth)
         spp = (time.perf_counter() - c0) / sub.size
-        cpu = {"value": round(1.0 / spp, 3), "unit": "probes/s", "cores": nth, "kind": "port",
-               "sample": f"FP32 C oracle, {sub.size} of the {P} probes (3.141 M rays each, per-probe cost "
-                         f"uniform in rays), {nth} threads, {spp * sub.size:.1f} s"}
+        cpu = cpu_line(round(1.0 / spp, 3), "probes/s",
+                       f"FP32 C oracle, {sub.size} of the {P} probes (3.141 M rays each, per-probe cost "
+                       f"uniform in rays), {nth} threads, {spp * sub.size:.1f} s")
     rays = cnt["primary_rays"] + cnt["shadow_rays"]
     return {
         "metric": METRIC, "value": round(P / el, 1), "unit": "probes/s", "n_gpus": 1,
@@ -845,7 +941,7 @@ def bench_bounce(args):
         err_line = None
         if oscene is not None:   # CPU leg: this bounce against the FP64 oracle fed the same grid
             ref = oscene.bake_indirect(psub, g, prev, lp, 1.0, mode64=True,
-                                       nthreads=min(16, os.cpu_count() or 1))["coeffs"]
+                                       nthreads=CPU_THREADS)["coeffs"]
             err = rel_err(cur[sub], ref)
             err_line = {"max": float(err.max()), "p99": float(np.percentile(err, 99)),
                         "probes": int(sub.size)}
@@ -895,26 +991,31 @@ def bench_copy(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2",
-                    choices=["c2", "c1", "c3", "c4", "c5", "copy", "sweep", "bounce", "reference"])
-    ap.add_argument("--rate", type=float, default=40.0, help="c5: request arrivals per second")
-    ap.add_argument("--policy", default="srpt", choices=sorted(SCHED), help="c5: server step scheduling")
-    ap.add_argument("--requests", type=int, default=200, help="c5: number of requests")
-    ap.add_argument("--state-budget", type=int, default=256, help="c5: server state arena (MiB)")
-    ap.add_argument("--step-rays", type=int, default=1 << 25, help="c5: primary rays per step")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c4",
+                    choices=["c4", "c2", "c1", "c3", "c5", "copy", "sweep", "bounce", "reference"])
+    ap.add_argument("--rate", type=float, default=40.0, help="served form: request arrivals per second")
+    ap.add_argument("--policy", default="srpt", choices=sorted(SCHED), help="served form: step scheduling")
+    ap.add_argument("--requests", type=int, default=200,
+                    help="served form: number of requests (0 skips the served leg of the default line)")
+    ap.add_argument("--state-budget", type=int, default=256, help="served form: state arena (MiB)")
+    ap.add_argument("--step-rays", type=int, default=1 << 25, help="served form: primary rays per step")
+    ap.add_argument("--plain-passes", type=int, default=2,
+                    help="batched form: timed passes of the plain per-job baker")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline: seconds of oracle work")
     ap.add_argument("--precision", default="fp32", choices=sorted(PREC))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--variant", type=int, default=0, help="trace kernel variant (0 = default)")
+    ap.add_argument("--variant", type=int, default=0, help="trace kernel variant (0 = autotuned / default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: the probe baker has no multi-GPU path in this "
-              "build (probes shard trivially; see the design notes).  Run with --gpus 1.",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: every box here has one GPU and the probe baker has no "
+              "multi-GPU path in this build (probes shard by probe group; see the design notes).  "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-    line = {"c2": bench_c2, "c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
-            "copy": bench_copy, "sweep": sweep_c2, "bounce": bench_bounce, "reference": bench_reference}[args.config](args)
+    line = {"c4": bench_c4, "c2": bench_c2, "c1": bench_c1, "c3": bench_c3, "c5": bench_c5,
+            "copy": bench_copy, "sweep": sweep_c2, "bounce": bench_bounce,
+            "reference": bench_reference}[args.config](args)
     print(json.dumps(line), flush=True)
 
 

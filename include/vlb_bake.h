@@ -48,10 +48,19 @@ enum {
     VLB_ERR_NOT_FOUND   = 5,  /* unknown scene / ticket / plan                */
     VLB_ERR_BUSY        = 6,  /* ticket not finished yet (poll)               */
     VLB_ERR_UNSUPPORTED = 7,  /* option not supported by this build           */
-    VLB_ERR_TIMEOUT     = 8,  /* wait deadline expired                        */
-    VLB_ERR_CANCELLED   = 9,  /* request cancelled before completion          */
+    VLB_ERR_TIMEOUT     = 8,  /* vlb_wait_for deadline passed                 */
+    VLB_ERR_CANCELLED   = 9,  /* request cancelled (vlb_cancel, or pending at */
+                              /* vlb_server_destroy) before it finished      */
     VLB_ERR_INJECTED    = 10, /* failure injected through vlb_debug_inject    */
     VLB_ERR_NO_DEVICE   = 11  /* no HIP device visible                        */
+};
+
+/* Failure-injection sites of vlb_debug_inject. */
+enum {
+    VLB_INJECT_PLAN_EXECUTE  = 0, /* vlb_plan_execute, vlb_bake               */
+    VLB_INJECT_BATCH_EXECUTE = 1, /* vlb_batch_execute                        */
+    VLB_INJECT_SERVER_STEP   = 2, /* one server step: its requests fail       */
+    VLB_INJECT_SITES         = 3
 };
 
 /* ----------------------------------------------------------------- options */
@@ -216,6 +225,14 @@ vlb_status  vlb_ctx_destroy(vlb_ctx* ctx);
 const char* vlb_last_error(const vlb_ctx* ctx);
 /* Number of compute units and device name (for reports). */
 vlb_status  vlb_device_info(vlb_ctx* ctx, int32_t* num_cus, char* name, size_t name_len);
+/* Failure injection for tests of callers' error handling (no counterpart in
+ * the reference, whose only failure path is an exception caught in main,
+ * src/baker/main.cpp:26-40): the next `count` calls at `site` (VLB_INJECT_*)
+ * fail with VLB_ERR_INJECTED before they launch or allocate anything; count 0
+ * disarms the site.  A server step that takes an injected failure fails the
+ * requests whose work it held (their vlb_wait returns VLB_ERR_INJECTED) and
+ * the server keeps serving the others. */
+vlb_status  vlb_debug_inject(vlb_ctx* ctx, int32_t site, int32_t count);
 
 /* ------------------------------------------------------------- scene API */
 /* Replaces SceneManager::pushScene + Scene_t::buildAccelerationStructures
@@ -229,6 +246,9 @@ vlb_status  vlb_scene_upload(vlb_ctx* ctx, const float* xyz, uint32_t nverts,
                              const uint32_t* idx, uint32_t ntris,
                              const float* normals, const float* albedo_rgb,
                              uint32_t* out_scene);
+/* Frees the scene's device buffers.  VLB_ERR_INVALID (and nothing freed)
+ * while plans, batch jobs or unfinished server requests use the scene:
+ * destroy or finish those first. */
 vlb_status  vlb_scene_release(vlb_ctx* ctx, uint32_t scene);
 vlb_status  vlb_scene_info_get(vlb_ctx* ctx, uint32_t scene, vlb_scene_info* out);
 /* Copies the BVH out: nodes as 16 floats each (child boxes + child ids, see
@@ -327,6 +347,11 @@ typedef struct vlb_batch_info {
     uint64_t device_bytes;    /* job state + tables + round buffers       */
     double   total_ms;        /* mean device time of the timed executes   */
     uint32_t jobs, rounds, grid, trace_variant, chunk, timed_runs;
+    /* with vlb_batch_set_timing(b, 1): mean device time per execute of the */
+    /* trace, projection and merge launches (summed over the rounds)        */
+    double   trace_ms, project_ms, merge_ms;
+    uint32_t kernel_timed_runs;
+    uint32_t reserved;
 } vlb_batch_info;
 vlb_status  vlb_batch_create(vlb_ctx* ctx, const vlb_batch_job* jobs, uint32_t njobs,
                              const vlb_batch_options* opt /* may be NULL */,
@@ -336,6 +361,12 @@ vlb_status  vlb_batch_execute(vlb_batch* b);
 vlb_status  vlb_batch_sync(vlb_batch* b);
 /* Capture execute() as one hipGraph and replay it from then on. */
 vlb_status  vlb_batch_use_graph(vlb_batch* b, int32_t on);
+/* Per-kernel timing: from now on every execute also records hipEvents
+ * around each round's trace, projection and merge launches (also inside a
+ * captured graph, as event-record nodes); vlb_batch_info reports the means.
+ * Each call resets the means; switching it on or off drops a captured graph
+ * (the next execute captures again). */
+vlb_status  vlb_batch_set_timing(vlb_batch* b, int32_t on);
 /* Coefficients of job `job`: [P][K][3] in the job's probe order. */
 vlb_status  vlb_batch_download(vlb_batch* b, uint32_t job, float* out, size_t out_len);
 vlb_status  vlb_batch_download_f64(vlb_batch* b, uint32_t job, double* out, size_t out_len);
@@ -359,8 +390,9 @@ vlb_status  vlb_batch_destroy(vlb_batch* b);
  * state_budget arena, and runs steps of at most step_rays primary rays that
  * hold a share of every admitted request's remaining work, two steps in
  * flight (see VLB_SCHED_*).  Each request has the semantics of vlb_bake and its coefficients
- * do not depend on what else is served.  Scenes must stay resident while
- * requests on them are pending.  vlb_submit copies everything it needs. */
+ * do not depend on what else is served.  A request holds its scene from
+ * vlb_submit until it is done (vlb_scene_release refuses meanwhile).
+ * vlb_submit copies everything it needs. */
 typedef struct vlb_server vlb_server;
 typedef struct vlb_server_options {
     uint64_t state_budget;    /* device bytes of admitted request state;  */
@@ -411,11 +443,32 @@ vlb_status  vlb_submit(vlb_server* s, uint32_t scene, const vlb_bake_params* p,
 /* *done = 1 once the request's coefficients are on the host. */
 vlb_status  vlb_poll(vlb_server* s, uint64_t request, int32_t* done);
 /* Blocks until the request is done, copies its [P][K][3] coefficients to
- * out (may be NULL), fills *info (may be NULL) and forgets the request. */
+ * out (may be NULL), fills *info (may be NULL) and forgets the request.
+ * Returns the request's status: VLB_OK, or why it did not finish
+ * (VLB_ERR_CANCELLED, VLB_ERR_INJECTED, a device error). */
 vlb_status  vlb_wait(vlb_server* s, uint64_t request, float* out, size_t out_len,
                      vlb_request_info* info);
+/* vlb_wait with a deadline: VLB_ERR_TIMEOUT if the request is not done
+ * within timeout_ms milliseconds of the call (0: just look); the request
+ * then stays and can be waited for again.  The reference waits on its fences
+ * without bound and only prints a failed wait (src/application.cpp:263-276). */
+vlb_status  vlb_wait_for(vlb_server* s, uint64_t request, double timeout_ms, float* out,
+                         size_t out_len, vlb_request_info* info);
+/* Cancels a request that has not finished: a queued one is dropped at once;
+ * an admitted one gets no further steps and releases its device state when
+ * the steps already launched with its work retire.  Its vlb_wait returns
+ * VLB_ERR_CANCELLED.  A request that finished first keeps its result (the
+ * call returns VLB_OK either way; vlb_wait tells which happened). */
+vlb_status  vlb_cancel(vlb_server* s, uint64_t request);
+/* Admission hold: while on, queued requests are not admitted (admitted ones
+ * keep running); off resumes admission.  For draining a server and for
+ * tests that need requests to stay queued. */
+vlb_status  vlb_server_hold(vlb_server* s, int32_t on);
 vlb_status  vlb_server_stats_get(vlb_server* s, vlb_server_stats* out);
-/* Finishes every submitted request, then stops the scheduler. */
+/* Stops the scheduler: requests still queued or running are cancelled (their
+ * work in flight on the device is waited for, not abandoned), results not yet
+ * collected are dropped, and every resource of the server is freed.  Collect
+ * the results you need with vlb_wait first. */
 vlb_status  vlb_server_destroy(vlb_server* s);
 
 /* --------------------------------------------------- SH of images / skies */

@@ -15,7 +15,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "vulkan-light-bakery_amd"))
 import vlb  # noqa: E402  (for the shared parameter struct layout)
 
-ORC_PATH = os.path.join(ROOT, "oracle", "liborc.so")
+# VLB_ORC_LIB selects another build of the same sources (bench.py times a
+# -march=native build made on the machine it runs on)
+ORC_PATH = os.environ.get("VLB_ORC_LIB") or os.path.join(ROOT, "oracle", "liborc.so")
 
 
 def _load():

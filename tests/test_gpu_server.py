@@ -63,9 +63,10 @@ def _bits(a):
     return np.ascontiguousarray(a).view(np.uint32)
 
 
-# 0: the default wavefront kernels; 12 and 3: the fused kernel (8 and 6
-# waves/SIMD)
-@pytest.mark.parametrize("variant", [0, 12, 3])
+# 0: the server's default (the fused kernel, variant 12); 12 and 3: the
+# fused kernel at 8 waves/SIMD and at the compiler's occupancy; 20: the
+# wavefront multi-job kernels (k_rays_jobs / k_shade_jobs, lane map 3)
+@pytest.mark.parametrize("variant", [0, 12, 3, 20])
 def test_server_request_equals_single_bake(ctx, scenes, variant):
     reqs = _requests(scenes, n=3)
     srv = vlb.Server(ctx, step_rays=1 << 16, trace_variant=variant)
@@ -77,7 +78,8 @@ def test_server_request_equals_single_bake(ctx, scenes, variant):
 
 
 @pytest.mark.parametrize("policy,variant", [(vlb.SCHED_SRPT, 0), (vlb.SCHED_SHARE, 0),
-                                            (vlb.SCHED_FIFO, 0), (vlb.SCHED_SRPT, 12)])
+                                            (vlb.SCHED_FIFO, 0), (vlb.SCHED_SRPT, 12),
+                                            (vlb.SCHED_SRPT, 20), (vlb.SCHED_SHARE, 20)])
 def test_server_alone_equals_in_mix(ctx, scenes, policy, variant):
     reqs = _requests(scenes, n=12, seed=5)
     target = 4
@@ -138,5 +140,5 @@ def test_server_rejects_bad_requests_and_drains_on_destroy(ctx, scenes):
             break
     got, info = srv.wait(rid)
     assert info["status"] == 0
-    srv.submit(sid, _params(_probes(20, 3)))   # pending at destroy: finished, then dropped
+    srv.submit(sid, _params(_probes(20, 3)))   # pending at destroy: cancelled
     srv.destroy()

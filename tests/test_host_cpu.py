@@ -40,6 +40,21 @@ def test_context_fails_loudly_without_device():
     assert e.value.status == vlb.ERR_NO_DEVICE
 
 
+def test_failure_path_entry_points_reject_bad_handles():
+    """Host-side argument checks of the failure-handling entry points: no
+    device call is made for a null handle or a bad site / deadline."""
+    L = vlb.lib
+    assert L.vlb_debug_inject(None, vlb.INJECT_PLAN_EXECUTE, 1) == vlb.ERR_INVALID
+    assert L.vlb_cancel(None, 1) == vlb.ERR_INVALID
+    assert L.vlb_server_hold(None, 1) == vlb.ERR_INVALID
+    assert L.vlb_wait_for(None, 1, 10.0, None, 0, None) == vlb.ERR_INVALID
+    assert L.vlb_batch_set_timing(None, 1) == vlb.ERR_INVALID
+    assert L.vlb_scene_release(None, 1) == vlb.ERR_INVALID
+    assert vlb.STATUS_NAMES[vlb.ERR_TIMEOUT] == "TIMEOUT"
+    assert vlb.STATUS_NAMES[vlb.ERR_CANCELLED] == "CANCELLED"
+    assert vlb.STATUS_NAMES[vlb.ERR_INJECTED] == "INJECTED"
+
+
 def test_probe_grid_reference_bfs_order():
     # light_baker.cpp:80-101 with probesCount3D = 7: first 7 probes walk x,
     # the next 42 walk y = 1..6 for each x, the rest walk z = 1..6 for each

@@ -451,6 +451,13 @@ vlb_status vlb_ctx_destroy(vlb_ctx* h)
     return VLB_OK;
 }
 
+vlb_status vlb_debug_inject(vlb_ctx* h, int32_t site, int32_t count)
+{
+    if (!h || site < 0 || site >= VLB_INJECT_SITES || count < 0) return VLB_ERR_INVALID;
+    reinterpret_cast<Context*>(h)->inject[site].store(count);
+    return VLB_OK;
+}
+
 const char* vlb_last_error(const vlb_ctx* h)
 {
     if (!h) return "null context";
@@ -502,7 +509,11 @@ vlb_status vlb_scene_release(vlb_ctx* h, uint32_t scene)
     Context* c = reinterpret_cast<Context*>(h);
     std::lock_guard<std::recursive_mutex> lk(c->mtx);
     return guard(c, [&] {
-        find_scene(*c, scene);
+        Scene& s = find_scene(*c, scene);
+        const int users = s.users.load();
+        VLB_REQUIRE(users == 0, VLB_ERR_INVALID,
+                    "scene " + std::to_string(scene) + " is in use by " + std::to_string(users) +
+                        " plans, batch jobs or unfinished server requests: destroy or finish them first");
         VLB_HIP(hipStreamSynchronize(c->stream));
         c->scenes.erase(scene);
     });
@@ -629,6 +640,7 @@ vlb_status vlb_plan_create(vlb_ctx* h, uint32_t scene, const vlb_bake_params* p,
         auto pl = std::make_unique<Plan>();
         build_plan(*c, s, *p, *pl);
         *out = reinterpret_cast<vlb_plan*>(pl.release());
+        ++s.users;
         ++c->children;
     });
 }
@@ -638,7 +650,11 @@ vlb_status vlb_plan_execute(vlb_plan* h)
     if (!h) return VLB_ERR_INVALID;
     Plan* pl = reinterpret_cast<Plan*>(h);
     std::lock_guard<std::recursive_mutex> lk(pl->ctx->mtx);
-    return guard(pl->ctx, [&] { plan_launch(*pl); });
+    return guard(pl->ctx, [&] {
+        VLB_REQUIRE(!take_injected(*pl->ctx, VLB_INJECT_PLAN_EXECUTE), VLB_ERR_INJECTED,
+                    "vlb_plan_execute: injected failure");
+        plan_launch(*pl);
+    });
 }
 
 vlb_status vlb_plan_sync(vlb_plan* h)
@@ -826,6 +842,7 @@ vlb_status vlb_plan_destroy(vlb_plan* h)
     Context* c = pl->ctx;
     std::lock_guard<std::recursive_mutex> lk(c->mtx);
     (void)hipStreamSynchronize(c->stream);
+    --pl->scene->users;
     delete pl;
     --c->children;
     return VLB_OK;
@@ -838,6 +855,8 @@ vlb_status vlb_bake(vlb_ctx* h, uint32_t scene, const vlb_bake_params* p, float*
     std::lock_guard<std::recursive_mutex> lk(c->mtx);
     return guard(c, [&] {
         Scene& s = find_scene(*c, scene);
+        VLB_REQUIRE(!take_injected(*c, VLB_INJECT_PLAN_EXECUTE), VLB_ERR_INJECTED,
+                    "vlb_bake: injected failure");
         VLB_HIP(hipSetDevice(c->device));
         Plan pl;
         build_plan(*c, s, *p, pl);

@@ -265,6 +265,7 @@ struct Batch {
     RoundBufs rb;
     uint32_t variant = kBatchDefaultVariant;
     int map = 3, refill = 32, chunk = 256, focc = -1;
+    std::vector<Scene*> scenes;    // one per job, each holding a use of its scene
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -273,11 +274,31 @@ struct Batch {
     uint32_t pending = 0;
     double sum_ms = 0.0;
     uint32_t runs = 0;
+    // per-kernel timing: 4 events per round (before the trace launch, then
+    // after the trace, projection and merge launches).  Eager executes use a
+    // ring of kKRing event sets; graph executes alternate between two graph
+    // instances, each captured with its own event set as event-record nodes,
+    // so the host reads one instance's events while the other runs.
+    static constexpr int kKRing = 32;
+    bool ktiming = false;
+    std::vector<hipEvent_t> kev;
+    uint32_t kpending = 0;
+    struct GraphSlot {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        bool unread = false;
+    };
+    GraphSlot gs[2];
+    uint32_t gnext = 0;
+    double ksum[3] = {0.0, 0.0, 0.0};
+    uint32_t kruns = 0;
     vlb_batch_info info{};
     ~Batch()
     {
         drop_graph();
         for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : kev)
             if (e) (void)hipEventDestroy(e);
     }
     void drop_graph()
@@ -286,21 +307,44 @@ struct Batch {
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;
         graph = nullptr;
+        for (GraphSlot& g : gs) {
+            if (g.x) (void)hipGraphExecDestroy(g.x);
+            if (g.g) (void)hipGraphDestroy(g.g);
+            g = GraphSlot();
+        }
+        gnext = 0;
     }
+    size_t kset() const { return rounds.size() * 4; }
     MultiArgs args(size_t r) const
     {
         const size_t off[3] = {round_off[r] + parts[r][0], round_off[r] + parts[r][1],
                                round_off[r] + parts[r][2]};
         return rb.args(jobtab.as<JobDesc>(), table.as<uint8_t>(), off, map, refill, chunk, focc);
     }
-    void launch_all(bool count, hipStream_t st)
+    // E: kset() events to record around the kernels, or null
+    void launch_all(bool count, hipStream_t st, hipEvent_t* E = nullptr)
     {
         for (size_t r = 0; r < rounds.size(); ++r) {
             const MultiArgs a = args(r);
+            if (E) VLB_HIP(hipEventRecord(E[4 * r + 0], st));
             launch_multi_trace(a, count, rb.grid, rb.shade_grid, st);
+            if (E) VLB_HIP(hipEventRecord(E[4 * r + 1], st));
             launch_multi_project(a, precision, rb.pgrid, st);
+            if (E) VLB_HIP(hipEventRecord(E[4 * r + 2], st));
             launch_multi_merge(a, rb.mgrid, st);
+            if (E) VLB_HIP(hipEventRecord(E[4 * r + 3], st));
         }
+    }
+    // Adds one execute's kernel times (its event set E, already complete).
+    void read_kernels(const hipEvent_t* E)
+    {
+        for (size_t r = 0; r < rounds.size(); ++r)
+            for (int k = 0; k < 3; ++k) {
+                float ms = 0.f;
+                VLB_HIP(hipEventElapsedTime(&ms, E[4 * r + k], E[4 * r + k + 1]));
+                ksum[k] += ms;
+            }
+        ++kruns;
     }
     void drain()
     {
@@ -311,34 +355,77 @@ struct Batch {
             ++runs;
         }
         pending = 0;
+        for (uint32_t i = 0; i < kpending; ++i) read_kernels(&kev[kset() * i]);
+        kpending = 0;
+        for (int i = 0; i < 2; ++i) {
+            const int j = (int)((gnext + i) & 1);  // launch order
+            if (gs[j].unread) {
+                read_kernels(&kev[kset() * j]);
+                gs[j].unread = false;
+            }
+        }
     }
     void sync()
     {
         VLB_HIP(hipStreamSynchronize(ctx->stream));
         drain();
     }
+    void capture(hipStream_t st, hipEvent_t* E, hipGraph_t& g, hipGraphExec_t& x)
+    {
+        VLB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            launch_all(false, st, E);
+        } catch (...) {
+            hipGraph_t gg = nullptr;
+            (void)hipStreamEndCapture(st, &gg);
+            if (gg) (void)hipGraphDestroy(gg);
+            throw;
+        }
+        VLB_HIP(hipStreamEndCapture(st, &g));
+        VLB_HIP(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+    }
+    void set_timing(bool on)
+    {
+        sync();
+        if (on != ktiming) drop_graph();  // captured with or without the events
+        ktiming = on;
+        ksum[0] = ksum[1] = ksum[2] = 0.0;
+        kruns = 0;
+        if (on && kev.size() < kset() * kKRing) {
+            for (auto& e : kev)
+                if (e) (void)hipEventDestroy(e);
+            kev.assign(kset() * kKRing, nullptr);
+            for (auto& e : kev) VLB_HIP(hipEventCreate(&e));
+        }
+    }
     void execute()
     {
         hipStream_t st = ctx->stream;
         if (pending == (uint32_t)kRing) sync();
         VLB_HIP(hipEventRecord(ev[2 * pending], st));
-        if (use_graph) {
-            if (!gexec) {
-                VLB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                try {
-                    launch_all(false, st);
-                } catch (...) {
-                    hipGraph_t g = nullptr;
-                    (void)hipStreamEndCapture(st, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    throw;
-                }
-                VLB_HIP(hipStreamEndCapture(st, &graph));
-                VLB_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        if (use_graph && ktiming) {
+            GraphSlot& g = gs[gnext];
+            hipEvent_t* E = &kev[kset() * gnext];
+            if (g.unread) {  // this instance's last replay: read before it replays
+                VLB_HIP(hipEventSynchronize(E[kset() - 1]));
+                read_kernels(E);
+                g.unread = false;
             }
+            if (!g.x) capture(st, E, g.g, g.x);
+            VLB_HIP(hipGraphLaunch(g.x, st));
+            g.unread = true;
+            gnext ^= 1u;
+        } else if (use_graph) {
+            if (!gexec) capture(st, nullptr, graph, gexec);
             VLB_HIP(hipGraphLaunch(gexec, st));
         } else {
-            launch_all(false, st);
+            hipEvent_t* E = nullptr;
+            if (ktiming) {
+                if (kpending == (uint32_t)kKRing) sync();
+                E = &kev[kset() * kpending];
+                ++kpending;
+            }
+            launch_all(false, st, E);
         }
         VLB_HIP(hipEventRecord(ev[2 * pending + 1], st));
         ++pending;
@@ -370,7 +457,9 @@ void build_batch(Context& ctx, const vlb_batch_job* jobs, uint32_t njobs, const 
         auto js = std::make_unique<JobState>();
         init_job(ctx, b.cache, jobs[j].scene, *jobs[j].params, b.keep64, *js);
         nvis = std::max(nvis, js->desc.nvis);
-        depth = std::max(depth, find_scene(ctx, jobs[j].scene).depth);
+        Scene& sc = find_scene(ctx, jobs[j].scene);
+        depth = std::max(depth, sc.depth);
+        b.scenes.push_back(&sc);
         max_nch = std::max(max_nch, js->nch);
         b.info.primary_rays += js->rays;
         b.info.projection_tiles += (uint64_t)js->ntiles;
@@ -424,6 +513,7 @@ void build_batch(Context& ctx, const vlb_batch_job* jobs, uint32_t njobs, const 
     b.info.jobs = njobs;
     b.info.rounds = (uint32_t)b.rounds.size();
     VLB_HIP(hipStreamSynchronize(ctx.stream));
+    for (Scene* sc : b.scenes) ++sc->users;  // built: the jobs hold their scenes
 }
 
 // A sample of the batch for the autotuner: every k-th projection tile of
@@ -479,6 +569,8 @@ vlb_status vlb_batch_execute(vlb_batch* h)
     Batch* b = as_batch(h);
     std::lock_guard<std::recursive_mutex> lk(b->ctx->mtx);
     return guard(b->ctx, [&] {
+        VLB_REQUIRE(!take_injected(*b->ctx, VLB_INJECT_BATCH_EXECUTE), VLB_ERR_INJECTED,
+                    "vlb_batch_execute: injected failure");
         VLB_HIP(hipSetDevice(b->ctx->device));
         b->execute();
     });
@@ -499,7 +591,21 @@ vlb_status vlb_batch_use_graph(vlb_batch* h, int32_t on)
     std::lock_guard<std::recursive_mutex> lk(b->ctx->mtx);
     return guard(b->ctx, [&] {
         b->use_graph = on != 0;
-        if (!b->use_graph) b->drop_graph();
+        if (!b->use_graph) {
+            b->sync();  // reads the kernel events of the graph's last replays
+            b->drop_graph();
+        }
+    });
+}
+
+vlb_status vlb_batch_set_timing(vlb_batch* h, int32_t on)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Batch* b = as_batch(h);
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mtx);
+    return guard(b->ctx, [&] {
+        VLB_HIP(hipSetDevice(b->ctx->device));
+        b->set_timing(on != 0);
     });
 }
 
@@ -632,6 +738,12 @@ vlb_status vlb_batch_info_get(vlb_batch* h, vlb_batch_info* out)
         out->grid = (uint32_t)b->rb.grid;
         out->trace_variant = b->variant;
         out->chunk = (uint32_t)b->chunk;
+        out->kernel_timed_runs = b->kruns;
+        if (b->kruns) {
+            out->trace_ms = b->ksum[0] / b->kruns;
+            out->project_ms = b->ksum[1] / b->kruns;
+            out->merge_ms = b->ksum[2] / b->kruns;
+        }
         size_t bytes = b->jobtab.bytes + b->table.bytes + b->cache.bytes();
         for (const auto& js : b->jobs) bytes += js->state_bytes();
         bytes += b->rb.rad.bytes + b->rb.hits.bytes + b->rb.vis.bytes + b->rb.spill.bytes;
@@ -646,6 +758,7 @@ vlb_status vlb_batch_destroy(vlb_batch* h)
     Context* c = b->ctx;
     std::lock_guard<std::recursive_mutex> lk(c->mtx);
     (void)hipStreamSynchronize(c->stream);
+    for (Scene* sc : b->scenes) --sc->users;
     delete b;
     --c->children;
     return VLB_OK;

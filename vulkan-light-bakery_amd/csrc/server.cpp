@@ -16,7 +16,12 @@
 //    are on the GPU starts at the next step and its work overlaps theirs;
 //  * two steps in flight: while the device runs step n, the host admits and
 //    stages step n+1 (its tables go through pinned staging buffers), and
-//    retires step n-1 (finished requests release their state).
+//    retires step n-1 (finished requests release their state);
+//  * failure handling: a request can be cancelled (dropped from the queue, or
+//    given no further steps once admitted), waited for with a deadline, and a
+//    step that fails before launch (an injected failure) fails only the
+//    requests whose work it held.  A request holds a use of its scene from
+//    submit until it is done, so the scene cannot be released under it.
 // One hipGraph, captured once, runs every step: the step's counts live in
 // device memory.  A request's coefficients are a function of the request
 // alone (fixed tile -> partial-sum slot map, fixed merge order), so they
@@ -97,6 +102,7 @@ enum ReqState { QUEUED, ACTIVE, DONE };
 struct Request {
     uint64_t id = 0;
     uint32_t scene = 0;
+    Scene* sc = nullptr;                  // held scene (users raised) until done
     vlb_bake_params params{};
     std::vector<float> probes;
     std::vector<vlb_light> lights;
@@ -110,12 +116,18 @@ struct Request {
     size_t res_off = 0, res_bytes = 0;   // pinned result arena block
     JobState js;                          // desc points into the arena block
     int next_tile = 0;                    // next projection tile to schedule
+    // failure handling
+    bool cancel = false;                  // vlb_cancel (or destroy) asked for it
+    bool abandoned = false;               // admitted, then dropped: no more steps
+    vlb_status fail = VLB_OK;             // why it was abandoned
+    int inflight = 0;                     // launched steps holding its work
 };
 
 struct InFlight {
     bool used = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::vector<Request*> finishing;      // requests whose merge is in the step
+    std::vector<Request*> members;        // every request with work in the step
 };
 
 struct Upload {
@@ -132,6 +144,7 @@ struct Server {
     std::mutex mu;
     std::condition_variable cv_sched, cv_done;
     bool stop = false;
+    bool hold = false;                    // vlb_server_hold: no admission
     std::deque<Request*> queue;
     std::vector<Request*> active;         // admission order
     std::map<uint64_t, std::unique_ptr<Request>> reqs;
@@ -170,9 +183,72 @@ struct Server {
     bool admit_one(Request* r, size_t& upload_used, std::vector<Upload>& ups, uint8_t* stg);
     void retire(InFlight& f);
     void launch_step(const RoundTable& rt, const std::vector<Upload>& ups, uint8_t* stg,
-                     InFlight& f, const std::vector<Request*>& finishing);
+                     InFlight& f, const std::vector<Request*>& finishing,
+                     const std::vector<Request*>& members);
     void fail_all(vlb_status st, const std::string& msg);
+    // with mu held:
+    void finish(Request* r, vlb_status st);
+    void abandon(Request* r, vlb_status st);
+    void drop_cancelled();
 };
+
+// Ends a request (with mu held): releases its device state, slot and scene
+// use, and, unless it succeeded, its result block; wakes its waiters.
+void Server::finish(Request* r, vlb_status st)
+{
+    if (r->slot >= 0) {
+        dev_arena.release(r->dev_off, r->dev_bytes);
+        free_slots.push_back(r->slot);
+        r->slot = -1;
+    }
+    if (st != VLB_OK && r->res_bytes) {
+        res_arena.release(r->res_off, r->res_bytes);
+        r->res_bytes = 0;
+    }
+    r->state = DONE;
+    r->status = st;
+    r->done_ms = now_ms();
+    r->js.rt.reset();
+    if (r->sc) {
+        --r->sc->users;
+        r->sc = nullptr;
+    }
+    if (st == VLB_OK) {
+        ++completed;
+        rays += r->js.rays;
+        probes += r->params.num_probes;
+    }
+    cv_done.notify_all();
+}
+
+// An admitted request gets no further steps (with mu held); it ends once
+// the steps already launched with its work have retired.
+void Server::abandon(Request* r, vlb_status st)
+{
+    if (r->abandoned || r->state == DONE) return;
+    auto it = std::find(active.begin(), active.end(), r);
+    if (it != active.end()) active.erase(it);
+    r->abandoned = true;
+    r->fail = st;
+    if (r->inflight == 0) finish(r, st);
+}
+
+// Cancellations asked for since the last step (with mu held).
+void Server::drop_cancelled()
+{
+    if (stop) {
+        while (!queue.empty()) {
+            Request* r = queue.front();
+            queue.pop_front();
+            finish(r, VLB_ERR_CANCELLED);
+        }
+        for (Request* r : active) r->cancel = true;
+    }
+    std::vector<Request*> drop;
+    for (Request* r : active)
+        if (r->cancel) drop.push_back(r);
+    for (Request* r : drop) abandon(r, VLB_ERR_CANCELLED);
+}
 
 // Device block of one request: probes | perm | partials | out.
 struct BlockLayout {
@@ -285,25 +361,20 @@ void Server::retire(InFlight& f)
     VLB_HIP(hipEventElapsedTime(&ms, f.e0, f.e1));
     std::lock_guard<std::mutex> lk(mu);
     busy_ms += ms;
-    const double t = now_ms();
-    for (Request* r : f.finishing) {
-        dev_arena.release(r->dev_off, r->dev_bytes);
-        free_slots.push_back(r->slot);
-        r->slot = -1;
-        r->state = DONE;
-        r->done_ms = t;
-        ++completed;
-        rays += r->js.rays;
-        probes += r->params.num_probes;
-        r->js.rt.reset();
-    }
+    for (Request* r : f.members) --r->inflight;
+    // a request cancelled after its last step was launched does not finish
+    for (Request* r : f.finishing) finish(r, r->cancel ? VLB_ERR_CANCELLED : VLB_OK);
+    for (Request* r : f.members)
+        if (r->abandoned && r->inflight == 0 && r->state != DONE) finish(r, r->fail);
     f.finishing.clear();
+    f.members.clear();
     f.used = false;
     cv_done.notify_all();
 }
 
 void Server::launch_step(const RoundTable& rt, const std::vector<Upload>& ups, uint8_t* stg,
-                         InFlight& f, const std::vector<Request*>& finishing)
+                         InFlight& f, const std::vector<Request*>& finishing,
+                         const std::vector<Request*>& members)
 {
     hipStream_t st = stream;
     memcpy(stg, jobtab.data(), jobtab_bytes);
@@ -335,22 +406,25 @@ void Server::launch_step(const RoundTable& rt, const std::vector<Upload>& ups, u
         VLB_HIP(hipMemcpyAsync(res_host + r->res_off, r->js.desc.out, r->res_bytes, hipMemcpyDeviceToHost, st));
     VLB_HIP(hipEventRecord(f.e1, st));
     f.finishing = finishing;
+    f.members = members;
     f.used = true;
     ++steps;
+    std::lock_guard<std::mutex> lk(mu);
+    for (Request* r : members) ++r->inflight;
 }
 
 void Server::fail_all(vlb_status st, const std::string& msg)
 {
+    // work already on the device may still read the requests' scenes: let it
+    // end before their uses are given back
+    (void)hipStreamSynchronize(stream);
+    (void)hipGetLastError();
     std::lock_guard<std::mutex> lk(mu);
     err_status = st;
     error = msg;
     for (auto& kv : reqs) {
         Request* r = kv.second.get();
-        if (r->state != DONE) {
-            r->state = DONE;
-            r->status = st;
-            r->done_ms = now_ms();
-        }
+        if (r->state != DONE) finish(r, st);
     }
     queue.clear();
     active.clear();
@@ -368,7 +442,8 @@ void Server::run()
             bool work;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                work = !queue.empty();
+                drop_cancelled();
+                work = !hold && !stop && !queue.empty();
                 for (Request* r : active) work = work || r->next_tile < r->js.ntiles;
                 if (!work && !next.used && !last.used) {
                     if (stop) break;
@@ -389,27 +464,25 @@ void Server::run()
             const size_t upl0 = jobtab_bytes + round_cap_bytes;
             std::vector<Upload> ups;
             RoundTable rt;
-            std::vector<Request*> finishing;
+            std::vector<Request*> finishing, members, admitted;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 size_t used = 0;
-                while (!queue.empty()) {
+                while (!hold && !stop && !queue.empty()) {
                     Request* r = queue.front();
                     bool ok = false;
                     try {
                         ok = admit_one(r, used, ups, stg + upl0);
                     } catch (const Error& e) {
-                        // this request only (e.g. its scene was released)
+                        // this request only (e.g. its ray table did not fit)
                         queue.pop_front();
-                        r->state = DONE;
-                        r->status = e.code;
-                        r->done_ms = now_ms();
-                        cv_done.notify_all();
+                        finish(r, e.code);
                         continue;
                     }
                     if (!ok) break;
                     queue.pop_front();
                     active.push_back(r);
+                    admitted.push_back(r);
                 }
                 for (Upload& u : ups) u.src += upl0;
                 active_peak = std::max(active_peak, (uint32_t)active.size());
@@ -454,12 +527,21 @@ void Server::run()
                     r->next_tile += (int)give[i];
                     if (r->start_ms < 0) r->start_ms = t;
                     ++r->steps;
+                    members.push_back(r);
                     if (r->next_tile == r->js.ntiles) {
                         round_merge(rt, r->slot, r->js);
                         finishing.push_back(r);
                     }
                 }
                 for (Request* r : finishing) active.erase(std::find(active.begin(), active.end(), r));
+                if (rt.hdr.nseg > 0 && take_injected(*ctx, VLB_INJECT_SERVER_STEP)) {
+                    // the step fails before launch: so do the requests it held
+                    // and those admitted with it (their probes were never
+                    // uploaded)
+                    for (Request* r : members) abandon(r, VLB_ERR_INJECTED);
+                    for (Request* r : admitted) abandon(r, VLB_ERR_INJECTED);
+                    continue;
+                }
             }
             if (rt.hdr.nseg == 0) {
                 // nothing admitted or runnable: the state or result arena is
@@ -472,7 +554,7 @@ void Server::run()
                 }
                 continue;
             }
-            launch_step(rt, ups, stg, next, finishing);
+            launch_step(rt, ups, stg, next, finishing, members);
             ++step_no;
         }
     } catch (const Error& e) {
@@ -612,6 +694,15 @@ vlb_status vlb_submit(vlb_server* h, uint32_t scene, const vlb_bake_params* p, u
         r->params.lights = r->lights.data();
         std::lock_guard<std::mutex> lk(s->mu);
         VLB_REQUIRE(s->err_status == VLB_OK, s->err_status, "server failed: " + s->error);
+        VLB_REQUIRE(!s->stop, VLB_ERR_INVALID, "server is being destroyed");
+        {
+            // the request holds its scene until it is done; raised under the
+            // context's mutex so that a release cannot slip in between
+            std::lock_guard<std::recursive_mutex> lk2(s->ctx->mtx);
+            Scene& sc = find_scene(*s->ctx, scene);
+            ++sc.users;
+            r->sc = &sc;
+        }
         r->id = s->next_id++;
         r->submit_ms = s->now_ms();
         if (request) *request = r->id;
@@ -633,7 +724,9 @@ vlb_status vlb_poll(vlb_server* h, uint64_t request, int32_t* done)
     return VLB_OK;
 }
 
-vlb_status vlb_wait(vlb_server* h, uint64_t request, float* out, size_t out_len, vlb_request_info* info)
+// timeout_ms < 0: no deadline
+static vlb_status wait_request(vlb_server* h, uint64_t request, double timeout_ms, float* out,
+                               size_t out_len, vlb_request_info* info)
 {
     if (!h) return VLB_ERR_INVALID;
     Server* s = as_server(h);
@@ -641,7 +734,14 @@ vlb_status vlb_wait(vlb_server* h, uint64_t request, float* out, size_t out_len,
     auto it = s->reqs.find(request);
     if (it == s->reqs.end()) return VLB_ERR_NOT_FOUND;
     Request* r = it->second.get();
-    s->cv_done.wait(lk, [&] { return r->state == DONE; });
+    if (timeout_ms < 0) {
+        s->cv_done.wait(lk, [&] { return r->state == DONE; });
+    } else {
+        const auto deadline = std::chrono::steady_clock::now() +
+                              std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                  std::chrono::duration<double, std::milli>(std::min(timeout_ms, 1e12)));
+        if (!s->cv_done.wait_until(lk, deadline, [&] { return r->state == DONE; })) return VLB_ERR_TIMEOUT;
+    }
     vlb_status st = r->status;
     const size_t n = (size_t)r->params.num_probes * (r->params.sh_order == 2 ? 9 : 16) * 3;
     if (st == VLB_OK && out) {
@@ -661,6 +761,46 @@ vlb_status vlb_wait(vlb_server* h, uint64_t request, float* out, size_t out_len,
     s->reqs.erase(it);
     s->cv_sched.notify_one();
     return st;
+}
+
+vlb_status vlb_wait(vlb_server* h, uint64_t request, float* out, size_t out_len, vlb_request_info* info)
+{
+    return wait_request(h, request, -1.0, out, out_len, info);
+}
+
+vlb_status vlb_wait_for(vlb_server* h, uint64_t request, double timeout_ms, float* out, size_t out_len,
+                        vlb_request_info* info)
+{
+    if (!(timeout_ms >= 0.0)) return VLB_ERR_INVALID;  // negative or NaN
+    return wait_request(h, request, timeout_ms, out, out_len, info);
+}
+
+vlb_status vlb_cancel(vlb_server* h, uint64_t request)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Server* s = as_server(h);
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(request);
+    if (it == s->reqs.end()) return VLB_ERR_NOT_FOUND;
+    Request* r = it->second.get();
+    if (r->state == QUEUED) {
+        s->queue.erase(std::find(s->queue.begin(), s->queue.end(), r));
+        s->finish(r, VLB_ERR_CANCELLED);
+    } else if (r->state == ACTIVE) {
+        r->cancel = true;  // the scheduler drops it before its next step
+        s->cv_sched.notify_one();
+    }
+    return VLB_OK;
+}
+
+vlb_status vlb_server_hold(vlb_server* h, int32_t on)
+{
+    if (!h) return VLB_ERR_INVALID;
+    Server* s = as_server(h);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->hold = on != 0;
+    s->cv_sched.notify_one();
+    return VLB_OK;
 }
 
 vlb_status vlb_server_stats_get(vlb_server* h, vlb_server_stats* out)

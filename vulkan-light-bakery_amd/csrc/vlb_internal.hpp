@@ -92,6 +92,11 @@ struct Scene {
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
     uint32_t depth = 0;
     double build_ms = 0.0;
+    // plans, batch jobs and server requests that read this scene's device
+    // buffers; vlb_scene_release refuses while it is non-zero.  Raised under
+    // the context's mutex (so a release cannot slip in between the lookup
+    // and the raise), lowered from any thread.
+    std::atomic<int> users{0};
     size_t bytes() const { return nodes.bytes + nodes4.bytes + qnodes4.bytes + tris.bytes + shade.bytes; }
     SceneView view() const
     {
@@ -133,7 +138,19 @@ struct Context {
     // accounting, so the context outlives them (guarded by mtx)
     int children = 0;
     std::recursive_mutex mtx;
+    // failures injected by vlb_debug_inject: calls left to fail per site
+    std::atomic<int> inject[VLB_INJECT_SITES] = {};
 };
+
+// Takes one injected failure of `site` if any is armed: the caller then
+// fails with VLB_ERR_INJECTED before it launches or allocates anything.
+inline bool take_injected(Context& ctx, int site)
+{
+    int n = ctx.inject[site].load();
+    while (n > 0)
+        if (ctx.inject[site].compare_exchange_weak(n, n - 1)) return true;
+    return false;
+}
 
 // ------------------------------------------------------- shared tables
 // Direction set of a bake in traversal order (Hilbert curve over the

@@ -40,6 +40,7 @@ REF_SRGB, REF_UNORM8 = 1, 2
 EQUIRECT_SH_COMP, EQUIRECT_SKYBOX_COMP = 0, 1
 SCENE_ROOM, SCENE_TERRAIN, SCENE_CITY, SCENE_CUBE, SCENE_SPHERES = range(5)
 SCHED_SRPT, SCHED_SHARE, SCHED_FIFO = 0, 1, 2
+INJECT_PLAN_EXECUTE, INJECT_BATCH_EXECUTE, INJECT_SERVER_STEP = 0, 1, 2
 MAX_LIGHTS = 8
 
 
@@ -102,7 +103,9 @@ class BatchInfo(C.Structure):
                 ("projection_tiles", C.c_uint64), ("device_bytes", C.c_uint64),
                 ("total_ms", C.c_double),
                 ("jobs", C.c_uint32), ("rounds", C.c_uint32), ("grid", C.c_uint32),
-                ("trace_variant", C.c_uint32), ("chunk", C.c_uint32), ("timed_runs", C.c_uint32)]
+                ("trace_variant", C.c_uint32), ("chunk", C.c_uint32), ("timed_runs", C.c_uint32),
+                ("trace_ms", C.c_double), ("project_ms", C.c_double), ("merge_ms", C.c_double),
+                ("kernel_timed_runs", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class ServerOptions(C.Structure):
@@ -179,6 +182,7 @@ def _load(path: str = LIB_PATH) -> C.CDLL:
         "vlb_ctx_destroy": (I32, [VP]),
         "vlb_last_error": (C.c_char_p, [VP]),
         "vlb_device_info": (I32, [VP, P(I32), C.c_char_p, SZ]),
+        "vlb_debug_inject": (I32, [VP, I32, I32]),
         "vlb_scene_upload": (I32, [VP, F, U32, P(U32), U32, F, F, P(U32)]),
         "vlb_scene_release": (I32, [VP, U32]),
         "vlb_scene_info_get": (I32, [VP, U32, P(SceneInfo)]),
@@ -202,6 +206,7 @@ def _load(path: str = LIB_PATH) -> C.CDLL:
         "vlb_batch_execute": (I32, [VP]),
         "vlb_batch_sync": (I32, [VP]),
         "vlb_batch_use_graph": (I32, [VP, I32]),
+        "vlb_batch_set_timing": (I32, [VP, I32]),
         "vlb_batch_download": (I32, [VP, U32, F, SZ]),
         "vlb_batch_download_f64": (I32, [VP, U32, P(C.c_double), SZ]),
         "vlb_batch_autotune": (I32, [VP, U32, P(U32), P(U32)]),
@@ -212,6 +217,9 @@ def _load(path: str = LIB_PATH) -> C.CDLL:
         "vlb_submit": (I32, [VP, U32, P(BakeParams), P(U64)]),
         "vlb_poll": (I32, [VP, U64, P(I32)]),
         "vlb_wait": (I32, [VP, U64, F, SZ, P(RequestInfo)]),
+        "vlb_wait_for": (I32, [VP, U64, C.c_double, F, SZ, P(RequestInfo)]),
+        "vlb_cancel": (I32, [VP, U64]),
+        "vlb_server_hold": (I32, [VP, I32]),
         "vlb_server_stats_get": (I32, [VP, P(ServerStats)]),
         "vlb_server_destroy": (I32, [VP]),
         "vlb_skybox_sh": (I32, [VP, F, U32, U32, U32, F, SZ]),
@@ -555,6 +563,11 @@ class Batch:
     def use_graph(self, on=True):
         self._chk(lib.vlb_batch_use_graph(self.h, int(on)), "batch_use_graph")
 
+    def set_timing(self, on=True):
+        """Per-kernel hipEvent timing of every later execute (trace_ms,
+        project_ms, merge_ms in info())."""
+        self._chk(lib.vlb_batch_set_timing(self.h, int(on)), "batch_set_timing")
+
     def download(self, job: int) -> np.ndarray:
         out = np.zeros(self.shapes[job], np.float32)
         self._chk(lib.vlb_batch_download(self.h, job, _fptr(out), out.size), "batch_download")
@@ -627,14 +640,34 @@ class Server:
             raise VlbError(st, "poll")
         return bool(d.value)
 
-    def wait(self, rid: int):
+    def wait(self, rid: int, timeout_ms: Optional[float] = None):
+        """Coefficients and info of a finished request.  With timeout_ms,
+        raises VlbError(ERR_TIMEOUT) if it is not done by then (the request
+        stays and can be waited for again).  A request that did not finish
+        raises VlbError with its status (ERR_CANCELLED, ERR_INJECTED, ...)."""
         out = np.zeros(self._shape[rid], np.float32)
         info = RequestInfo()
-        st = lib.vlb_wait(self.h, rid, _fptr(out), out.size, C.byref(info))
+        if timeout_ms is None:
+            st = lib.vlb_wait(self.h, rid, _fptr(out), out.size, C.byref(info))
+        else:
+            st = lib.vlb_wait_for(self.h, rid, float(timeout_ms), _fptr(out), out.size, C.byref(info))
+            if st == ERR_TIMEOUT:
+                raise VlbError(st, "wait_for: deadline passed")
         del self._shape[rid]
         if st:
             raise VlbError(st, f"wait: {self.ctx.last_error()}")
         return out, {k: getattr(info, k) for k, _ in RequestInfo._fields_}
+
+    def cancel(self, rid: int):
+        st = lib.vlb_cancel(self.h, rid)
+        if st:
+            raise VlbError(st, "cancel")
+
+    def hold(self, on=True):
+        """Admission hold: queued requests stay queued while on."""
+        st = lib.vlb_server_hold(self.h, int(on))
+        if st:
+            raise VlbError(st, "server_hold")
 
     def stats(self) -> dict:
         s = ServerStats()
@@ -688,6 +721,10 @@ class Context:
     def _chk(self, st, what):
         if st:
             raise VlbError(st, f"{what}: {self.last_error()}")
+
+    def debug_inject(self, site: int, count: int = 1):
+        """The next `count` calls at `site` (INJECT_*) fail with ERR_INJECTED."""
+        self._chk(lib.vlb_debug_inject(self.h, site, count), "debug_inject")
 
     def device_info(self):
         n = C.c_int32(0)
