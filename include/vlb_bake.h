@@ -487,6 +487,21 @@ vlb_status  vlb_skybox_sh(vlb_ctx* ctx, const float* cubemaps, uint32_t face_siz
 vlb_status  vlb_skybox_sh_d(vlb_ctx* ctx, const void* d_cubemaps, uint32_t face_size,
                             uint32_t batch, uint32_t sh_order, void* d_out);
 vlb_status  vlb_skybox_last_ms(vlb_ctx* ctx, double* project_ms, double* merge_ms);
+/* Allocation-free variant of vlb_skybox_sh_d.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc/hipFree or other allocating routine: its scratch
+ * (the face's solid-angle and direction tables and the per-workgroup partial
+ * sums) is a fixed device array of VLB_SKYBOX_NOALLOC_SCRATCH bytes that
+ * each GPU holds until the process ends, shared by every call on that GPU:
+ * calls that overlap on one GPU race.  Requests whose scratch would exceed it
+ * fail with VLB_ERR_INVALID and launch nothing (1024 cubemaps of 6 x 128^2
+ * need about 2 MiB; one of 6 x 1024^2 about 4.4 MiB).  d_cubemaps, d_out:
+ * device pointers as for vlb_skybox_sh_d; asynchronous on the context
+ * stream; no event timing. */
+#define VLB_SKYBOX_NOALLOC_SCRATCH (8u << 20)
+vlb_status  vlb_skybox_sh_noalloc(vlb_ctx* ctx, const void* d_cubemaps, uint32_t face_size,
+                                  uint32_t batch, uint32_t sh_order, void* d_out);
 
 /* Projection of an equirect RGBA8 image to 16 SH coefficients x RGB:
  * variant VLB_EQUIRECT_SH_COMP is shaders/sh.comp:23-42 (host

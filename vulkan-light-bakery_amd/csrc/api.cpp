@@ -416,6 +416,7 @@ vlb_status vlb_ctx_create(int32_t device, uint64_t hbm_budget_bytes, vlb_ctx** o
         c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)freeb;
         VLB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto& e : c->ev) VLB_HIP(hipEventCreate(&e));
+        c->sky_scratch = skybox_scratch_init(device);
     });
     if (s != VLB_OK) {
         delete c;
@@ -953,6 +954,38 @@ vlb_status vlb_skybox_sh_d(vlb_ctx* h, const void* d_cubes, uint32_t face_size, 
                          c->sky_tab.p ? c->sky_tab.as<float4>() : nullptr, c->sky_partial.as<float>(), bpc, static_cast<float*>(d_out), c->stream,
                          c->ev[0], c->ev[1], c->ev[2]);
         c->sky_pending = true;
+    });
+}
+
+vlb_status vlb_skybox_sh_noalloc(vlb_ctx* h, const void* d_cubes, uint32_t face_size, uint32_t batch,
+                                 uint32_t sh_order, void* d_out)
+{
+    if (!h || !d_cubes || !d_out) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_REQUIRE(face_size >= 1 && face_size <= 8192, VLB_ERR_INVALID, "face_size out of range");
+        VLB_REQUIRE(batch >= 1 && (uint64_t)batch * 6 * face_size * face_size < (1ull << 31),
+                    VLB_ERR_INVALID, "batch out of range");
+        VLB_REQUIRE(sh_order == 2 || sh_order == 3, VLB_ERR_INVALID, "sh_order must be 2 or 3");
+        VLB_REQUIRE(c->sky_scratch, VLB_ERR_DEVICE, "skybox scratch not resolved for this device");
+        const int N = (int)face_size, K = sh_order == 2 ? 9 : 16;
+        const int bpc = skybox_blocks_per_cube(N, (int)batch);
+        // scratch layout: solid angles | directions (faces <= 147 texels) | partials
+        auto a256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        const size_t wbytes = a256((size_t)N * N * sizeof(float));
+        const size_t tab_bytes = (size_t)6 * N * N * sizeof(float4);
+        const bool use_tab = tab_bytes <= (2u << 20);
+        const size_t part_off = wbytes + (use_tab ? a256(tab_bytes) : 0);
+        const size_t need = part_off + (size_t)batch * bpc * K * 3 * sizeof(float);
+        VLB_REQUIRE(need <= kSkyScratchBytes, VLB_ERR_INVALID,
+                    "vlb_skybox_sh_noalloc: " + std::to_string(batch) + " cubemaps of " +
+                        std::to_string(N) + "^2 texels need " + std::to_string(need) +
+                        " bytes of scratch, more than its " + std::to_string(kSkyScratchBytes) +
+                        "; use vlb_skybox_sh_d");
+        VLB_HIP(hipSetDevice(c->device));
+        launch_skybox_sh_noalloc(static_cast<const float4*>(d_cubes), N, (int)batch, K, c->sky_scratch,
+                                 wbytes, part_off, use_tab, bpc, static_cast<float*>(d_out), c->stream);
     });
 }
 

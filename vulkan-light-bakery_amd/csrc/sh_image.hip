@@ -15,6 +15,8 @@
 #include "vlb_internal.hpp"
 #include "sh_eval.hpp"
 
+#include <mutex>
+
 namespace vlb {
 namespace {
 
@@ -285,6 +287,24 @@ __global__ __launch_bounds__(256) void k_sh_recon(const float* __restrict__ coef
     reinterpret_cast<uchar4*>(out)[p] = o;
 }
 
+// Exact solid angle of each texel of a cube-map face, the host table of
+// solid_angle_table (api.cpp) computed on the device in double: the
+// allocation-free entry point builds it in its scratch instead of uploading.
+__device__ __forceinline__ double corner_area(double x, double y)
+{
+    return atan2(x * y, sqrt(x * x + y * y + 1.0));
+}
+
+__global__ __launch_bounds__(256) void k_solid_angle(int N, float* __restrict__ w)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= N * N) return;
+    const int j = t / N, i = t - j * N;
+    const double x0 = 2.0 * i / N - 1.0, x1 = 2.0 * (i + 1) / N - 1.0;
+    const double y0 = 2.0 * j / N - 1.0, y1 = 2.0 * (j + 1) / N - 1.0;
+    w[t] = (float)(corner_area(x0, y0) - corner_area(x0, y1) - corner_area(x1, y0) + corner_area(x1, y1));
+}
+
 // Plain streaming copy, 16 B per lane per iteration (bandwidth reference).
 __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst,
                                               const float4* __restrict__ src, size_t n)
@@ -293,7 +313,32 @@ __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst,
         dst[i] = src[i];
 }
 
+// Scratch of the allocation-free skybox projection (vlb_skybox_sh_noalloc):
+// [solid-angle table | direction table | partial sums] of the current call.
+// The name is a host-side stand-in in host code, never a device address:
+// skybox_scratch_init resolves this device's address once, and launches take
+// that pointer.  Every call on a GPU shares the array (calls that overlap on
+// one GPU race) and overwrites all of it that it reads.
+static __device__ __align__(256) unsigned char scratch[kSkyScratchBytes];
+
 }  // namespace
+
+// Device address of `scratch` on `device` (the current device), resolved
+// with hipGetSymbolAddress the first time and kept per device.
+void* skybox_scratch_init(int device)
+{
+    static std::mutex mu;
+    static void* addr[64] = {};
+    VLB_REQUIRE(device >= 0 && device < 64, VLB_ERR_INVALID, "device index beyond 63");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        VLB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        VLB_REQUIRE(p != nullptr, VLB_ERR_DEVICE, "hipGetSymbolAddress returned no address");
+        addr[device] = p;
+    }
+    return addr[device];
+}
 
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
 {
@@ -341,6 +386,20 @@ void launch_skybox_sh(const float4* cubes, int N, int batch, int K, const float*
     const int nv = K * 3;
     launch_merge_partials(partial, batch, blocks_per_cube, nv, out, nullptr, st);
     if (e2) VLB_HIP(hipEventRecord(e2, st));
+}
+
+void launch_skybox_sh_noalloc(const float4* cubes, int N, int batch, int K, void* scr,
+                              size_t tab_off, size_t part_off, bool use_tab, int blocks_per_cube,
+                              float* out, hipStream_t st)
+{
+    uint8_t* base = static_cast<uint8_t*>(scr);
+    float* wtab = reinterpret_cast<float*>(base);
+    k_solid_angle<<<(N * N + 255) / 256, 256, 0, st>>>(N, wtab);
+    VLB_HIP(hipGetLastError());
+    float4* tab = use_tab ? reinterpret_cast<float4*>(base + tab_off) : nullptr;
+    if (tab) launch_skybox_table(N, wtab, tab, st);
+    launch_skybox_sh(cubes, N, batch, K, wtab, tab, reinterpret_cast<float*>(base + part_off),
+                     blocks_per_cube, out, st, nullptr, nullptr, nullptr);
 }
 
 void launch_equirect_project(const uint8_t* img, int W, int H, int variant, double* out,

@@ -140,6 +140,8 @@ struct Context {
     std::recursive_mutex mtx;
     // failures injected by vlb_debug_inject: calls left to fail per site
     std::atomic<int> inject[VLB_INJECT_SITES] = {};
+    // this device's address of the allocation-free skybox scratch
+    void* sky_scratch = nullptr;
 };
 
 // Takes one injected failure of `site` if any is armed: the caller then
@@ -202,6 +204,11 @@ inline vlb_status guard(Context* ctx, F&& f)
         return VLB_ERR_INVALID;
     }
 }
+
+// Scratch of vlb_skybox_sh_noalloc (sh_image.hip): solid-angle table,
+// direction table and partial sums of one call, in a device array each GPU
+// holds until the process ends.
+constexpr size_t kSkyScratchBytes = VLB_SKYBOX_NOALLOC_SCRATCH;
 
 // tables.cpp
 Scene& find_scene(Context& ctx, uint32_t id);
@@ -413,6 +420,10 @@ void launch_skybox_sh(const float4* cubes, int N, int batch, int K, const float*
                       const float4* tab,
                       float* partial, int blocks_per_cube, float* out, hipStream_t st,
                       hipEvent_t e0, hipEvent_t e1, hipEvent_t e2);
+void* skybox_scratch_init(int device);
+void launch_skybox_sh_noalloc(const float4* cubes, int N, int batch, int K, void* scratch,
+                              size_t tab_off, size_t part_off, bool use_tab, int blocks_per_cube,
+                              float* out, hipStream_t st);
 void launch_equirect_project(const uint8_t* img, int W, int H, int variant, double* out,
                              float* partial, int nblocks, hipStream_t st);
 void launch_sh_reconstruct(const float* coeffs, int W, int H, uint8_t* out, hipStream_t st);

@@ -224,6 +224,7 @@ def _load(path: str = LIB_PATH) -> C.CDLL:
         "vlb_server_destroy": (I32, [VP]),
         "vlb_skybox_sh": (I32, [VP, F, U32, U32, U32, F, SZ]),
         "vlb_skybox_sh_d": (I32, [VP, VP, U32, U32, U32, VP]),
+        "vlb_skybox_sh_noalloc": (I32, [VP, VP, U32, U32, U32, VP]),
         "vlb_skybox_last_ms": (I32, [VP, P(C.c_double), P(C.c_double)]),
         "vlb_project_equirect": (I32, [VP, P(C.c_uint8), U32, U32, I32, F]),
         "vlb_sh_reconstruct": (I32, [VP, F, U32, U32, P(C.c_uint8)]),
@@ -880,6 +881,12 @@ class Context:
     def skybox_sh_device(self, d_cubes: int, face_size: int, batch: int, sh_order: int, d_out: int):
         self._chk(lib.vlb_skybox_sh_d(self.h, C.c_void_p(d_cubes), face_size, batch, sh_order,
                                       C.c_void_p(d_out)), "skybox_sh_d")
+
+    def skybox_sh_noalloc(self, d_cubes: int, face_size: int, batch: int, sh_order: int, d_out: int):
+        """vlb_skybox_sh_noalloc: vlb_skybox_sh_d with no device allocation
+        (fixed per-GPU scratch; calls that overlap on one GPU race)."""
+        self._chk(lib.vlb_skybox_sh_noalloc(self.h, C.c_void_p(d_cubes), face_size, batch, sh_order,
+                                            C.c_void_p(d_out)), "skybox_sh_noalloc")
 
     def skybox_last_ms(self):
         a, b = C.c_double(0), C.c_double(0)
