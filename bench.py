@@ -137,7 +137,9 @@ def trace_roofline(kernel, variant, cnt_nodes, cnt_tris, launch_s, launches, pro
     duration, so frac <= 1 is a real HBM fraction.  `alg_rate_gbs` is the
     algorithmic node + triangle byte rate (112 or 64 B per BVH4 node visit +
     48 B per triangle test, counted by the kernels' counting pass): those
-    bytes are mostly served by L1/L2/MALL, so it can exceed the HBM peak."""
+    bytes are mostly served by L1/L2/MALL, so it can exceed the HBM peak.
+    `issue` (from the profile's SQ counters) is the bound that does hold:
+    the share of VALU issue slots used and the lanes busy per instruction."""
     alg = (node_bytes(variant) * cnt_nodes + TRI_BYTES * cnt_tris) / launches
     prof = None
     if profile:
@@ -150,6 +152,7 @@ def trace_roofline(kernel, variant, cnt_nodes, cnt_tris, launch_s, launches, pro
             "traffic": traffic, "kernel": kernel, "launch_ms": round(launch_s * 1e3, 4),
             "launches_per_step": launches, "alg_bytes_per_launch": round(alg, 1),
             "alg_rate_gbs": round(alg / launch_s / 1e9, 1),
+            "issue": prof.get("issue") if prof else None,
             "profile": (profile or {}).get("file") if prof else None}
 
 
@@ -339,33 +342,48 @@ def bench_c4(args):
         variant, chunk = b.autotune(2)
     b.count()
     cnt = b.info()
+    # timed region 1 (`value`): the batch as one captured hipGraph per step
     b.use_graph(True)
-    b.set_timing(True)
     for _ in range(max(1, args.warmup)):
         b.execute()
     b.sync()
-    b.set_timing(True)                    # the means cover the timed executes only
     t0 = time.perf_counter()
     for _ in range(args.steps):
         b.execute()
     b.sync()
     t1 = time.perf_counter()
-    info = b.info()
     el = (t1 - t0) / args.steps
+    graph_ms = b.info()["total_ms"]
     got = [b.download(j) for j in range(len(jobs16))]
+    # timed region 2 (kernel durations): the same steps launched eagerly with
+    # hipEvents around every kernel on the library stream (events inside a
+    # captured graph cannot be timed); same kernels, same work
+    b.use_graph(False)
+    b.set_timing(True)
+    b.execute()
+    b.sync()
+    b.set_timing(True)                    # the means cover the timed executes only
+    e0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.execute()
+    b.sync()
+    el_eager = (time.perf_counter() - e0) / args.steps
+    info = b.info()
     b.destroy()
 
     # the plain FP32 GPU baker: one synchronous vlb_bake per job, in order;
     # one untimed pass, then whole passes timed
     jobs32 = [(sids[1 + si], W.sun_params(pr, R, PREC["fp32"], sun)) for si, pr, R in spec]
-    for sid, prm in jobs32:
-        ctx.bake(sid, prm)
-    passes = max(1, args.plain_passes)
-    p0 = time.perf_counter()
-    for _ in range(passes):
+    passes = args.plain_passes
+    plain_s = None
+    if passes > 0:
         for sid, prm in jobs32:
             ctx.bake(sid, prm)
-    plain_s = (time.perf_counter() - p0) / passes
+        p0 = time.perf_counter()
+        for _ in range(passes):
+            for sid, prm in jobs32:
+                ctx.bake(sid, prm)
+        plain_s = (time.perf_counter() - p0) / passes
 
     # the served form, same context and resident scenes
     served = None
@@ -433,13 +451,15 @@ def bench_c4(args):
                     "triangles": round(cnt["tri_tests"] / rays, 2)},
         "kernel_ms": {"trace": round(info["trace_ms"], 3), "project": round(info["project_ms"], 3),
                       "merge": round(info["merge_ms"], 3), "timed_runs": info["kernel_timed_runs"],
-                      "device_total": round(info["total_ms"], 3)},
+                      "eager_step_ms": round(el_eager * 1e3, 2), "graph_device_ms": round(graph_ms, 3),
+                      "note": "per-kernel means over the timed eager steps (sum over the rounds)"},
         "sh_rel_err_vs_fp64": err,
         "hit_parity": hits,
         "roofline": roof,
-        "plain_fp32_baker": {"probes_per_s": round(P_all / plain_s, 1), "seconds_per_pass": round(plain_s, 4),
-                             "passes": passes, "calls_per_pass": len(jobs32)},
-        "speedup_vs_plain_fp32": round(plain_s / el, 3),
+        "plain_fp32_baker": None if plain_s is None else
+        {"probes_per_s": round(P_all / plain_s, 1), "seconds_per_pass": round(plain_s, 4),
+         "passes": passes, "calls_per_pass": len(jobs32), "warm": "one untimed pass first"},
+        "speedup_vs_plain_fp32": None if plain_s is None else round(plain_s / el, 3),
         "served": served,
         "cpu_baseline": cpu,
         "device_bytes": info["device_bytes"],
@@ -1002,7 +1022,7 @@ def main():
     ap.add_argument("--state-budget", type=int, default=256, help="served form: state arena (MiB)")
     ap.add_argument("--step-rays", type=int, default=1 << 25, help="served form: primary rays per step")
     ap.add_argument("--plain-passes", type=int, default=2,
-                    help="batched form: timed passes of the plain per-job baker")
+                    help="batched form: timed passes of the plain per-job baker (0 skips it)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline: seconds of oracle work")
     ap.add_argument("--precision", default="fp32", choices=sorted(PREC))
     ap.add_argument("--no-cpu-baseline", action="store_true")

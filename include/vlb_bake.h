@@ -361,11 +361,12 @@ vlb_status  vlb_batch_execute(vlb_batch* b);
 vlb_status  vlb_batch_sync(vlb_batch* b);
 /* Capture execute() as one hipGraph and replay it from then on. */
 vlb_status  vlb_batch_use_graph(vlb_batch* b, int32_t on);
-/* Per-kernel timing: from now on every execute also records hipEvents
- * around each round's trace, projection and merge launches (also inside a
- * captured graph, as event-record nodes); vlb_batch_info reports the means.
- * Each call resets the means; switching it on or off drops a captured graph
- * (the next execute captures again). */
+/* Per-kernel timing: from now on every eager execute (use_graph off) also
+ * records hipEvents around each round's trace, projection and merge launches
+ * on the context stream; vlb_batch_info reports the means per execute (summed
+ * over the rounds).  A graph replay is timed as a whole only: events recorded
+ * into a graph under capture cannot be timed on this runtime.  Each call
+ * resets the means. */
 vlb_status  vlb_batch_set_timing(vlb_batch* b, int32_t on);
 /* Coefficients of job `job`: [P][K][3] in the job's probe order. */
 vlb_status  vlb_batch_download(vlb_batch* b, uint32_t job, float* out, size_t out_len);

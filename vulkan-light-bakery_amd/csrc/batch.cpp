@@ -274,22 +274,15 @@ struct Batch {
     uint32_t pending = 0;
     double sum_ms = 0.0;
     uint32_t runs = 0;
-    // per-kernel timing: 4 events per round (before the trace launch, then
-    // after the trace, projection and merge launches).  Eager executes use a
-    // ring of kKRing event sets; graph executes alternate between two graph
-    // instances, each captured with its own event set as event-record nodes,
-    // so the host reads one instance's events while the other runs.
+    // per-kernel timing of eager executes: 4 events per round (before the
+    // trace launch, then after the trace, projection and merge launches), a
+    // ring of kKRing executes.  A captured graph is timed as a whole: events
+    // recorded into a graph under capture cannot be timed on this runtime
+    // (hipEventElapsedTime fails on them).
     static constexpr int kKRing = 32;
     bool ktiming = false;
     std::vector<hipEvent_t> kev;
     uint32_t kpending = 0;
-    struct GraphSlot {
-        hipGraph_t g = nullptr;
-        hipGraphExec_t x = nullptr;
-        bool unread = false;
-    };
-    GraphSlot gs[2];
-    uint32_t gnext = 0;
     double ksum[3] = {0.0, 0.0, 0.0};
     uint32_t kruns = 0;
     vlb_batch_info info{};
@@ -307,12 +300,6 @@ struct Batch {
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;
         graph = nullptr;
-        for (GraphSlot& g : gs) {
-            if (g.x) (void)hipGraphExecDestroy(g.x);
-            if (g.g) (void)hipGraphDestroy(g.g);
-            g = GraphSlot();
-        }
-        gnext = 0;
     }
     size_t kset() const { return rounds.size() * 4; }
     MultiArgs args(size_t r) const
@@ -357,24 +344,17 @@ struct Batch {
         pending = 0;
         for (uint32_t i = 0; i < kpending; ++i) read_kernels(&kev[kset() * i]);
         kpending = 0;
-        for (int i = 0; i < 2; ++i) {
-            const int j = (int)((gnext + i) & 1);  // launch order
-            if (gs[j].unread) {
-                read_kernels(&kev[kset() * j]);
-                gs[j].unread = false;
-            }
-        }
     }
     void sync()
     {
         VLB_HIP(hipStreamSynchronize(ctx->stream));
         drain();
     }
-    void capture(hipStream_t st, hipEvent_t* E, hipGraph_t& g, hipGraphExec_t& x)
+    void capture(hipStream_t st, hipGraph_t& g, hipGraphExec_t& x)
     {
         VLB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         try {
-            launch_all(false, st, E);
+            launch_all(false, st);
         } catch (...) {
             hipGraph_t gg = nullptr;
             (void)hipStreamEndCapture(st, &gg);
@@ -387,7 +367,6 @@ struct Batch {
     void set_timing(bool on)
     {
         sync();
-        if (on != ktiming) drop_graph();  // captured with or without the events
         ktiming = on;
         ksum[0] = ksum[1] = ksum[2] = 0.0;
         kruns = 0;
@@ -403,20 +382,8 @@ struct Batch {
         hipStream_t st = ctx->stream;
         if (pending == (uint32_t)kRing) sync();
         VLB_HIP(hipEventRecord(ev[2 * pending], st));
-        if (use_graph && ktiming) {
-            GraphSlot& g = gs[gnext];
-            hipEvent_t* E = &kev[kset() * gnext];
-            if (g.unread) {  // this instance's last replay: read before it replays
-                VLB_HIP(hipEventSynchronize(E[kset() - 1]));
-                read_kernels(E);
-                g.unread = false;
-            }
-            if (!g.x) capture(st, E, g.g, g.x);
-            VLB_HIP(hipGraphLaunch(g.x, st));
-            g.unread = true;
-            gnext ^= 1u;
-        } else if (use_graph) {
-            if (!gexec) capture(st, nullptr, graph, gexec);
+        if (use_graph) {
+            if (!gexec) capture(st, graph, gexec);
             VLB_HIP(hipGraphLaunch(gexec, st));
         } else {
             hipEvent_t* E = nullptr;

@@ -21,7 +21,9 @@ from typing import Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libvlb.so")
+# VLB_LIB selects another build of the same sources (the sanitizer build of
+# the CPU tests, vulkan-light-bakery_amd/Makefile asan-test)
+LIB_PATH = os.environ.get("VLB_LIB") or os.path.join(_HERE, "libvlb.so")
 
 # ---------------------------------------------------------------- constants
 OK, ERR_INVALID, ERR_DEVICE, ERR_OOM, ERR_BUDGET, ERR_NOT_FOUND, ERR_BUSY, \
