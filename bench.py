@@ -768,8 +768,10 @@ def bench_c3(args):
     runs = {}
     cnt = None
     for name, prec in (("fp32", 0), ("bf16", 1), ("fp16", 2)):
-        plan = ctx.plan(sid, params(prec, 3))
-        variant = plan.autotune(1)
+        prm = params(prec, 3)
+        prm.trace_variant = args.variant
+        plan = ctx.plan(sid, prm)
+        variant = plan.autotune(1) if args.variant == 0 else args.variant
         if cnt is None:
             plan.count()
             cnt = plan.stats()
