@@ -8,7 +8,8 @@
 // kernels walk.  Node boxes are unions of their children's boxes, i.e. the
 // exact min/max of their triangles' vertices (conservative for the
 // watertight test), and every step is deterministic.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "vlb_internal.hpp"
 
@@ -558,13 +559,11 @@ void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
                                       vals.as<uint32_t>());
     VLB_HIP(hipGetLastError());
     size_t tmp_bytes = 0;
-    VLB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.as<uint32_t>(),
-                                               keys2.as<uint32_t>(), vals.as<uint32_t>(),
-                                               vals2.as<uint32_t>(), n, 0, 30, st));
+    VLB_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys.as<uint32_t>(), keys2.as<uint32_t>(),
+                                      vals.as<uint32_t>(), vals2.as<uint32_t>(), (size_t)n, 0, 30, st));
     DevBuf tmp = dev_alloc(ctx, tmp_bytes > 0 ? tmp_bytes : 16);
-    VLB_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, keys.as<uint32_t>(),
-                                               keys2.as<uint32_t>(), vals.as<uint32_t>(),
-                                               vals2.as<uint32_t>(), n, 0, 30, st));
+    VLB_HIP(rocprim::radix_sort_pairs(tmp.p, tmp_bytes, keys.as<uint32_t>(), keys2.as<uint32_t>(),
+                                      vals.as<uint32_t>(), vals2.as<uint32_t>(), (size_t)n, 0, 30, st));
     // hierarchy (PLOC), one host round trip per pass for the cluster count
     const int nin = n > 1 ? n - 1 : 1;
     s.nodes = dev_alloc(ctx, (size_t)nin * sizeof(BNode));
@@ -593,8 +592,9 @@ void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
         DevBuf flag = dev_alloc(ctx, (size_t)n * sizeof(unsigned long long));
         DevBuf scan = dev_alloc(ctx, (size_t)n * sizeof(unsigned long long));
         size_t ps_bytes = 0;
-        VLB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, ps_bytes, flag.as<unsigned long long>(),
-                                                 scan.as<unsigned long long>(), n, st));
+        VLB_HIP(rocprim::inclusive_scan(nullptr, ps_bytes, flag.as<unsigned long long>(),
+                                        scan.as<unsigned long long>(), (size_t)n,
+                                        rocprim::plus<unsigned long long>(), st));
         DevBuf ps_tmp = dev_alloc(ctx, ps_bytes > 0 ? ps_bytes : 16);
         int m = n, cur = 0;
         // every pass merges at least one pair, so at most n - 1 passes
@@ -607,9 +607,9 @@ void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
                 k_ploc_nn<<<nblk(m), 256, 0, st>>>(clo[cur].as<float4>(), chi[cur].as<float4>(), m,
                                                    nn.as<int>());
             k_ploc_flags<<<nblk(m), 256, 0, st>>>(nn.as<int>(), m, flag.as<unsigned long long>());
-            VLB_HIP(hipcub::DeviceScan::InclusiveSum(ps_tmp.p, ps_bytes,
-                                                     flag.as<unsigned long long>(),
-                                                     scan.as<unsigned long long>(), m, st));
+            VLB_HIP(rocprim::inclusive_scan(ps_tmp.p, ps_bytes, flag.as<unsigned long long>(),
+                                            scan.as<unsigned long long>(), (size_t)m,
+                                            rocprim::plus<unsigned long long>(), st));
             k_ploc_merge<<<nblk(m), 256, 0, st>>>(
                 nn.as<int>(), m, scan.as<unsigned long long>(), clo[cur].as<float4>(),
                 chi[cur].as<float4>(), cid[cur].as<int>(), clo[cur ^ 1].as<float4>(),
@@ -645,10 +645,11 @@ void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
         VLB_HIP(hipGetLastError());
     }
     size_t scan_bytes = 0;
-    VLB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, even.as<int>(), idx4.as<int>(), nin, st));
+    VLB_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, even.as<int>(), idx4.as<int>(), 0, (size_t)nin,
+                                    rocprim::plus<int>(), st));
     DevBuf scan_tmp = dev_alloc(ctx, scan_bytes > 0 ? scan_bytes : 16);
-    VLB_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, even.as<int>(), idx4.as<int>(),
-                                             nin, st));
+    VLB_HIP(rocprim::exclusive_scan(scan_tmp.p, scan_bytes, even.as<int>(), idx4.as<int>(), 0, (size_t)nin,
+                                    rocprim::plus<int>(), st));
     int last[2] = {0, 0};
     VLB_HIP(hipMemcpyAsync(&last[0], idx4.as<int>() + (nin - 1), sizeof(int), hipMemcpyDeviceToHost, st));
     VLB_HIP(hipMemcpyAsync(&last[1], even.as<int>() + (nin - 1), sizeof(int), hipMemcpyDeviceToHost, st));
