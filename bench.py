@@ -694,6 +694,15 @@ def bench_c1(args):
     kms = float(np.mean(proj))
     traffic = load_traffic("skybox_c1_traffic.json")
     achieved = (nbytes + B * 27 * 4) / (kms / 1e3) / 1e9
+    # attainable read bandwidth on this device: a read-only streaming pass
+    # over the same cubemap buffer, at the skybox's own grid (3 workgroups
+    # per cubemap) and at 16 workgroups per CU; the faster one is the ceiling
+    ceil = {}
+    for grid in (3 * B, 0):
+        ctx.device_read(d_in, nbytes, grid)
+        ms = [ctx.device_read(d_in, nbytes, grid) for _ in range(5)]
+        ceil[grid] = nbytes / (float(np.mean(ms)) / 1e3) / 1e9
+    ceil_grid = max(ceil, key=ceil.get)
     line = {"metric": "skyboxes/s (6x128x128 RGBA32F -> L2)", "value": round(B * args.steps / (t1 - t0), 1),
             "unit": "skyboxes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(1e3 * (t1 - t0) / args.steps, 4), "higher_is_better": True,
@@ -705,7 +714,10 @@ def bench_c1(args):
                        "merge_us": round(1e3 * float(np.median(m1)), 2), "calls": len(lat)},
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                         "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None},
+                         "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None,
+                         "attainable_read_gbs": round(ceil[ceil_grid], 1),
+                         "attainable_grid": ceil_grid or "16 per CU",
+                         "frac_of_attainable": round(achieved / ceil[ceil_grid], 4)},
             "cpu_baseline": cpu}
     ctx.close()
     return line
@@ -1002,10 +1014,14 @@ def bench_copy(args):
     ms = [ctx.device_copy(dst, src, nbytes) for _ in range(args.steps)]
     best = min(ms)
     mean = sum(ms) / len(ms)
+    ctx.device_read(src, nbytes)
+    rd = [ctx.device_read(src, nbytes) for _ in range(args.steps)]
+    read_gbs = nbytes / (sum(rd) / len(rd) / 1e3) / 1e9
     line = {"metric": "streaming copy bandwidth (read+write)", "value": round(2 * nbytes / (mean / 1e3) / 1e9, 1),
             "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(mean, 4), "best_ms": round(best, 4), "bytes_copied": nbytes,
-            "higher_is_better": True, "dtype": "u8", "config": {"workload": "k_copy 1 GiB"}}
+            "higher_is_better": True, "dtype": "u8", "config": {"workload": "k_copy 1 GiB"},
+            "read_only_gbs": round(read_gbs, 1)}
     ctx.close()
     return line
 

@@ -595,6 +595,13 @@ vlb_status  vlb_ctx_sync(vlb_ctx* ctx);
  * bandwidth.  *ms (may be NULL) gets the event-timed duration. */
 vlb_status  vlb_device_copy(vlb_ctx* ctx, void* dst, const void* src, size_t bytes,
                             double* ms);
+/* Read-only streaming pass over `bytes` (multiple of 16) of device memory,
+ * split into `grid` contiguous workgroup ranges (0: 16 per compute unit),
+ * on the context stream.  No counterpart in the reference: it measures the
+ * attainable read bandwidth that read-bound kernels (the skybox projection)
+ * are compared with.  *ms (may be NULL) gets the event-timed duration. */
+vlb_status  vlb_device_read(vlb_ctx* ctx, const void* src, size_t bytes, uint32_t grid,
+                            double* ms);
 
 /* ------------------------------------------------------------ host helpers */
 /* LightBaker::probePositionsFromBoudingBox (src/baker/light_baker.cpp:80-101):

@@ -1190,6 +1190,28 @@ vlb_status vlb_device_copy(vlb_ctx* h, void* dst, const void* src, size_t bytes,
     });
 }
 
+vlb_status vlb_device_read(vlb_ctx* h, const void* src, size_t bytes, uint32_t grid, double* ms)
+{
+    if (!h || !src) return VLB_ERR_INVALID;
+    Context* c = reinterpret_cast<Context*>(h);
+    std::lock_guard<std::recursive_mutex> lk(c->mtx);
+    return guard(c, [&] {
+        VLB_REQUIRE(bytes % 16 == 0, VLB_ERR_INVALID, "read size must be a multiple of 16");
+        VLB_REQUIRE(grid <= (1u << 20), VLB_ERR_INVALID, "grid above 2^20 workgroups");
+        VLB_HIP(hipSetDevice(c->device));
+        const int g = grid ? (int)grid : 16 * c->num_cus;
+        VLB_HIP(hipEventRecord(c->ev[0], c->stream));
+        launch_read(src, bytes, g, c->stream);
+        VLB_HIP(hipEventRecord(c->ev[1], c->stream));
+        if (ms) {
+            VLB_HIP(hipStreamSynchronize(c->stream));
+            float f = 0.f;
+            VLB_HIP(hipEventElapsedTime(&f, c->ev[0], c->ev[1]));
+            *ms = f;
+        }
+    });
+}
+
 vlb_status vlb_ctx_sync(vlb_ctx* h)
 {
     if (!h) return VLB_ERR_INVALID;

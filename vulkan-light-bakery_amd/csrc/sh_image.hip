@@ -313,6 +313,30 @@ __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst,
         dst[i] = src[i];
 }
 
+// Read-only streaming pass (bandwidth reference for read-bound kernels such
+// as k_skybox): each workgroup streams its own contiguous range with four
+// 16-byte nontemporal loads per lane per round, issued before they are
+// used, as k_skybox does.  The sum is handed to an empty asm statement,
+// which keeps the loads from being removed; nothing is stored.
+__global__ __launch_bounds__(256) void k_read(const float4* __restrict__ src, size_t n, size_t per_block)
+{
+    typedef float nf4 __attribute__((ext_vector_type(4)));
+    const size_t start = (size_t)blockIdx.x * per_block;
+    const size_t end = min(n, start + per_block);
+    nf4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+    size_t t = start + threadIdx.x;
+    for (; t + 768 < end; t += 1024) {
+        nf4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(src + t + 256 * u));
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a += q[u];
+    }
+    for (; t < end; t += 256) a += __builtin_nontemporal_load(reinterpret_cast<const nf4*>(src + t));
+    const float sum = (a.x + a.y) + (a.z + a.w);
+    __asm__ __volatile__("" ::"v"(sum));
+}
+
 // Scratch of the allocation-free skybox projection (vlb_skybox_sh_noalloc):
 // [solid-angle table | direction table | partial sums] of the current call.
 // The name is a host-side stand-in in host code, never a device address:
@@ -346,6 +370,16 @@ void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
     if (n == 0) return;
     const int grid = (int)std::min<size_t>((n + 255) / 256, 256 * 64);
     k_copy<<<grid, 256, 0, st>>>(static_cast<float4*>(dst), static_cast<const float4*>(src), n);
+    VLB_HIP(hipGetLastError());
+}
+
+void launch_read(const void* src, size_t bytes, int grid, hipStream_t st)
+{
+    const size_t n = bytes / 16;
+    if (n == 0) return;
+    const size_t per_block = ((n + grid - 1) / grid + 255) / 256 * 256;
+    const int g = (int)((n + per_block - 1) / per_block);
+    k_read<<<g, 256, 0, st>>>(static_cast<const float4*>(src), n, per_block);
     VLB_HIP(hipGetLastError());
 }
 

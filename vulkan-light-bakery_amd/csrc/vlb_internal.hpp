@@ -428,6 +428,7 @@ void launch_equirect_project(const uint8_t* img, int W, int H, int variant, doub
                              float* partial, int nblocks, hipStream_t st);
 void launch_sh_reconstruct(const float* coeffs, int W, int H, uint8_t* out, hipStream_t st);
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
+void launch_read(const void* src, size_t bytes, int grid, hipStream_t st);
 
 // host_util.cpp
 void fibonacci_dirs(uint32_t R, float* xyz);

@@ -238,6 +238,7 @@ def _load(path: str = LIB_PATH) -> C.CDLL:
         "vlb_download": (I32, [VP, VP, VP, SZ]),
         "vlb_ctx_sync": (I32, [VP]),
         "vlb_device_copy": (I32, [VP, VP, VP, SZ, P(C.c_double)]),
+        "vlb_device_read": (I32, [VP, VP, SZ, U32, P(C.c_double)]),
         "vlb_probe_grid": (I32, [F, F, P(U32), I32, F, F]),
         "vlb_directions": (I32, [I32, U32, U32, U32, F, F, SZ]),
         "vlb_base64_encode": (I32, [P(C.c_uint8), SZ, C.c_char_p, SZ, P(SZ)]),
@@ -935,6 +936,13 @@ class Context:
         ms = C.c_double(0)
         self._chk(lib.vlb_device_copy(self.h, C.c_void_p(dst), C.c_void_p(src), nbytes,
                                       C.byref(ms)), "device_copy")
+        return ms.value
+
+    def device_read(self, src: int, nbytes: int, grid: int = 0) -> float:
+        """Event-timed read-only streaming pass over nbytes at src (ms)."""
+        ms = C.c_double(0)
+        self._chk(lib.vlb_device_read(self.h, C.c_void_p(src), nbytes, grid, C.byref(ms)),
+                  "device_read")
         return ms.value
 
     def __del__(self):
