@@ -253,7 +253,8 @@ def serve(ctx, vlb, W, host, sids, args, oracle_of=None):
     n = args.requests
     arrive, reqs, sun0 = W.c5_requests(host, n, args.rate)
     srv = vlb.Server(ctx, precision=PREC["bf16"], state_budget=args.state_budget << 20,
-                     step_rays=args.step_rays, policy=SCHED[args.policy], trace_variant=args.variant)
+                     step_rays=args.step_rays, policy=SCHED[args.policy], trace_variant=args.variant,
+                     age_ms=args.age_ms)
     # warm-up: one small request per scene (direction tables, graph capture)
     for si in range(len(host)):
         w = (si, reqs[0][1][:256], sun0,
@@ -309,6 +310,7 @@ def serve(ctx, vlb, W, host, sids, args, oracle_of=None):
                       "active_peak": st["active_peak"], "state_bytes_peak": st["state_bytes_peak"],
                       "span_s": round(srv_span, 3), "policy": args.policy,
                       "state_budget_mib": args.state_budget, "step_rays": args.step_rays,
+                      "age_ms": args.age_ms,
                       "trace_variant": args.variant or 12},
            "plain_fp32_per_request": {"probes_per_s": round(P_all / plain_span, 1),
                                       "p50_ms": round(float(np.percentile(plat, 50)), 2),
@@ -1038,7 +1040,11 @@ def main():
     ap.add_argument("--requests", type=int, default=200,
                     help="served form: number of requests (0 skips the served leg of the default line)")
     ap.add_argument("--state-budget", type=int, default=256, help="served form: state arena (MiB)")
-    ap.add_argument("--step-rays", type=int, default=1 << 25, help="served form: primary rays per step")
+    ap.add_argument("--step-rays", type=int, default=1 << 23,
+                    help="served form: primary rays per step (2^23: half the median latency of 2^25 at "
+                         "the same p99 when arrival-bound)")
+    ap.add_argument("--age-ms", type=int, default=0,
+                    help="served form: SRPT age bound (ms since submit; 0 = none)")
     ap.add_argument("--plain-passes", type=int, default=2,
                     help="batched form: timed passes of the plain per-job baker (0 skips it)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline: seconds of oracle work")

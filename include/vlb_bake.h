@@ -410,12 +410,14 @@ typedef struct vlb_server_options {
     uint32_t chunk;           /* rays per work fetch; 0 = 256             */
     uint32_t max_depth;       /* deepest scene BVH served; 0 = 64         */
     uint32_t policy;          /* VLB_SCHED_*: how a step's rays are shared */
-    uint32_t reserved;
+    uint32_t age_ms;          /* SRPT: requests submitted more than age_ms */
+                              /* ago go first, oldest first; 0 = no bound */
 } vlb_server_options;
 /* Step scheduling: SRPT fills each step with the admitted requests that have
- * the least work left first (best median latency); SHARE gives every
- * admitted request an equal share (processor sharing); FIFO serves them in
- * arrival order. */
+ * the least work left first (best median latency), except that requests
+ * older than age_ms (when set) go first, oldest first, which bounds how long
+ * a large request can be passed over; SHARE gives every admitted request an
+ * equal share (processor sharing); FIFO serves them in arrival order. */
 enum { VLB_SCHED_SRPT = 0, VLB_SCHED_SHARE = 1, VLB_SCHED_FIFO = 2 };
 typedef struct vlb_request_info {
     double   submit_ms;       /* server clock: ms since vlb_server_create */

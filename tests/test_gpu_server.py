@@ -77,13 +77,16 @@ def test_server_request_equals_single_bake(ctx, scenes, variant):
     srv.destroy()
 
 
-@pytest.mark.parametrize("policy,variant", [(vlb.SCHED_SRPT, 0), (vlb.SCHED_SHARE, 0),
-                                            (vlb.SCHED_FIFO, 0), (vlb.SCHED_SRPT, 12),
-                                            (vlb.SCHED_SRPT, 20), (vlb.SCHED_SHARE, 20)])
-def test_server_alone_equals_in_mix(ctx, scenes, policy, variant):
+# age 1: SRPT with an age bound of 1 ms, so that requests pass from SRPT
+# order to oldest-first while the mix runs
+@pytest.mark.parametrize("policy,variant,age", [(vlb.SCHED_SRPT, 0, 0), (vlb.SCHED_SHARE, 0, 0),
+                                                (vlb.SCHED_FIFO, 0, 0), (vlb.SCHED_SRPT, 12, 0),
+                                                (vlb.SCHED_SRPT, 20, 0), (vlb.SCHED_SHARE, 20, 0),
+                                                (vlb.SCHED_SRPT, 0, 1)])
+def test_server_alone_equals_in_mix(ctx, scenes, policy, variant, age):
     reqs = _requests(scenes, n=12, seed=5)
     target = 4
-    srv = vlb.Server(ctx, step_rays=1 << 15, policy=policy, trace_variant=variant)
+    srv = vlb.Server(ctx, step_rays=1 << 15, policy=policy, trace_variant=variant, age_ms=age)
     alone, _ = srv.wait(srv.submit(*reqs[target]))
     ids = [srv.submit(sid, p) for sid, p in reqs]      # all in flight together
     res = [srv.wait(i) for i in ids]
@@ -96,6 +99,14 @@ def test_server_alone_equals_in_mix(ctx, scenes, policy, variant):
         ref = orc.Scene.from_dict(byid[sid]).bake(p, mode64=True)["coeffs"]
         assert rel_err(got, ref).max() <= 1e-5
     srv.destroy()
+
+
+def test_server_age_bound_is_for_srpt_only(ctx):
+    for policy in (vlb.SCHED_SHARE, vlb.SCHED_FIFO):
+        with pytest.raises(vlb.VlbError) as e:
+            vlb.Server(ctx, policy=policy, age_ms=50)
+        assert e.value.status == vlb.ERR_INVALID
+    vlb.Server(ctx, policy=vlb.SCHED_SRPT, age_ms=50).destroy()
 
 
 def test_server_admission_by_state_budget(ctx, scenes):

@@ -11,7 +11,8 @@
 //    the pinned result arena; nothing is allocated or freed with hipMalloc /
 //    hipFree while serving, so no step waits for the device to go idle;
 //  * continuous batching: every step is filled with the next projection tiles
-//    of the admitted requests -- least work left first (SRPT, the default),
+//    of the admitted requests -- least work left first (SRPT, the default;
+//    with an optional age bound past which a request goes first),
 //    equal shares, or arrival order -- so a request that arrives while others
 //    are on the GPU starts at the next step and its work overlaps theirs;
 //  * two steps in flight: while the device runs step n, the host admits and
@@ -495,9 +496,17 @@ void Server::run()
                 auto left_tiles = [](const Request* r) {
                     return (size_t)(r->js.ntiles - r->next_tile) * (size_t)r->js.nch;
                 };
-                if (opt.policy == VLB_SCHED_SRPT)
-                    std::stable_sort(runnable.begin(), runnable.end(),
-                                     [&](const Request* a, const Request* b) { return left_tiles(a) < left_tiles(b); });
+                if (opt.policy == VLB_SCHED_SRPT) {
+                    // requests past the age bound first, oldest first; then
+                    // least work left
+                    const double old_before = opt.age_ms ? now_ms() - (double)opt.age_ms : -1e300;
+                    std::stable_sort(runnable.begin(), runnable.end(), [&](const Request* a, const Request* b) {
+                        const bool oa = a->submit_ms < old_before, ob = b->submit_ms < old_before;
+                        if (oa != ob) return oa;
+                        if (oa) return a->submit_ms < b->submit_ms;
+                        return left_tiles(a) < left_tiles(b);
+                    });
+                }
                 std::vector<size_t> give(runnable.size(), 0);
                 size_t room = cap_tiles;
                 if (opt.policy == VLB_SCHED_SHARE && !runnable.empty()) {
@@ -622,6 +631,8 @@ vlb_status vlb_server_create(vlb_ctx* h, const vlb_server_options* o, vlb_server
                     "bad precision");
         VLB_REQUIRE(op.step_rays <= (1ull << 30), VLB_ERR_INVALID, "step_rays must be <= 2^30");
         VLB_REQUIRE(op.policy <= VLB_SCHED_FIFO, VLB_ERR_INVALID, "bad scheduling policy");
+        VLB_REQUIRE(op.age_ms == 0 || op.policy == VLB_SCHED_SRPT, VLB_ERR_INVALID,
+                    "age_ms applies to the SRPT policy only");
         multi_variant(op.trace_variant, s->map, s->refill, s->focc);
         s->chunk = (int)op.chunk;
         VLB_REQUIRE(s->chunk >= 64 && s->chunk <= 1024 && 1024 % s->chunk == 0, VLB_ERR_INVALID,

@@ -115,7 +115,7 @@ class ServerOptions(C.Structure):
                 ("result_bytes", C.c_uint64), ("upload_bytes", C.c_uint64),
                 ("max_active", C.c_uint32), ("max_lights", C.c_uint32), ("precision", C.c_int32),
                 ("trace_variant", C.c_uint32), ("chunk", C.c_uint32), ("max_depth", C.c_uint32),
-                ("policy", C.c_uint32), ("reserved", C.c_uint32)]
+                ("policy", C.c_uint32), ("age_ms", C.c_uint32)]
 
 
 class RequestInfo(C.Structure):
@@ -614,13 +614,13 @@ class Server:
     def __init__(self, ctx: "Context", precision: int = PREC_FP32, state_budget: int = 0,
                  step_rays: int = 0, max_active: int = 0, max_lights: int = 0,
                  trace_variant: int = 0, chunk: int = 0, result_bytes: int = 0,
-                 upload_bytes: int = 0, policy: int = 0):
+                 upload_bytes: int = 0, policy: int = 0, age_ms: int = 0):
         self.ctx = ctx
         o = ServerOptions()
         o.state_budget, o.step_rays, o.max_active = state_budget, step_rays, max_active
         o.max_lights, o.precision, o.trace_variant, o.chunk = max_lights, precision, trace_variant, chunk
         o.result_bytes, o.upload_bytes = result_bytes, upload_bytes
-        o.policy = policy
+        o.policy, o.age_ms = policy, age_ms
         self.h = C.c_void_p()
         st = lib.vlb_server_create(ctx.h, C.byref(o), C.byref(self.h))
         if st:
