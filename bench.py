@@ -1053,9 +1053,8 @@ def main():
     ap.add_argument("--variant", type=int, default=0, help="trace kernel variant (0 = autotuned / default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: every box here has one GPU and the probe baker has no "
-              "multi-GPU path in this build (probes shard by probe group; see the design notes).  "
-              "Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the probe baker has no multi-GPU path (probes would shard "
+              "by probe group, each GPU holding a BVH replica).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     line = {"c4": bench_c4, "c2": bench_c2, "c1": bench_c1, "c3": bench_c3, "c5": bench_c5,
             "copy": bench_copy, "sweep": sweep_c2, "bounce": bench_bounce,

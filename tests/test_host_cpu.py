@@ -146,3 +146,14 @@ def test_cxx_facade_reports_missing_device():
     assert r.returncode != 0 and "status 11" in r.stderr
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "Select scene" in r.stderr
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for argv, env in ((["--gpus", "2"], {}), ([], {"WORLD_SIZE": "4"})):
+        r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), *argv], cwd=root,
+                           capture_output=True, text=True, timeout=120, env={**os.environ, **env})
+        assert r.returncode == 2
+        assert "one GPU" in r.stderr and r.stdout == ""
