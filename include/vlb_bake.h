@@ -47,7 +47,7 @@ enum {
     VLB_ERR_BUDGET      = 4,  /* request exceeds the context's HBM budget     */
     VLB_ERR_NOT_FOUND   = 5,  /* unknown scene / ticket / plan                */
     VLB_ERR_BUSY        = 6,  /* ticket not finished yet (poll)               */
-    VLB_ERR_UNSUPPORTED = 7,  /* option not supported by this build           */
+    VLB_ERR_UNSUPPORTED = 7,  /* option not supported by this library         */
     VLB_ERR_TIMEOUT     = 8,  /* vlb_wait_for deadline passed                 */
     VLB_ERR_CANCELLED   = 9,  /* request cancelled (vlb_cancel, or pending at */
                               /* vlb_server_destroy) before it finished      */
@@ -475,7 +475,7 @@ vlb_status  vlb_server_stats_get(vlb_server* s, vlb_server_stats* out);
 vlb_status  vlb_server_destroy(vlb_server* s);
 
 /* --------------------------------------------------- SH of images / skies */
-/* Skybox SH of float cubemaps: the build's form of skybox_sh.comp
+/* Skybox SH of float cubemaps: this library's form of skybox_sh.comp
  * (shaders/skybox_sh.comp:23-42, host src/skybox_manager.cpp:107-130) for a
  * 6 x N x N RGBA32F cubemap with exact texel solid angles.  Faces +X,-X,+Y,
  * -Y,+Z,-Z in Vulkan cube-map orientation.  The context keeps the solid-angle

@@ -7,7 +7,7 @@
  * Reference: traceRayEXT of shaders/env_map.rgen:26 (closest hit, opaque,
  * tmin 0.001, tmax 1e4) and env_map.rchit:86-87 (TerminateOnFirstHit |
  * SkipClosestHit shadow query).  The reference's arithmetic lives in the
- * Vulkan driver and RT hardware; what this file fixes is the build's own
+ * Vulkan driver and RT hardware; what this file fixes is the library's
  * contract: the closest hit is the lexicographic minimum of (t, prim id)
  * over all triangles that the watertight test accepts in [tmin, tmax], and a
  * shadow query is "some triangle accepted".  Both are independent of the

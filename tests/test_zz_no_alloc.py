@@ -1,5 +1,5 @@
 """The allocation-free entry point vlb_skybox_sh_noalloc against the entry
-point it varies, vlb_skybox_sh_d (the build's form of skybox_sh.comp,
+point it varies, vlb_skybox_sh_d (the library's form of skybox_sh.comp,
 shaders/skybox_sh.comp:23-42), through the C ABI: two calls in a row, the
 second on a smaller input (the fixed scratch keeps the first call's
 contents), each within 1e-5 relative L2 of the allocating entry point and

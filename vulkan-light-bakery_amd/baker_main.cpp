@@ -1,6 +1,6 @@
 // baker -- command-line driver with the reference's shape (src/baker/main.cpp:
 // 8-44): construct LightBaker, bake(), serialize(); exceptions map to
-// EXIT_FAILURE.  Extra options select the build's bake forms.
+// EXIT_FAILURE.  Extra options select the library's other bake forms.
 //
 //   baker <scene.gltf|scene.glb|room|terrain|city|cube|spheres[:seed]>
 //         [--grid N] [--rays R | --equirect W H] [--lambert] [--order2]

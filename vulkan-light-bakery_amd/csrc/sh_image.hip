@@ -1,6 +1,6 @@
 // sh_image.hip -- SH projection of skies and images.
 //
-//  k_skybox    : the build's form of skybox_sh.comp (shaders/skybox_sh.comp:
+//  k_skybox    : this library's form of skybox_sh.comp (shaders/skybox_sh.comp:
 //                23-42, host src/skybox_manager.cpp:107-130): batched
 //                6 x N x N RGBA32F cubemaps -> K coefs x RGB, exact texel
 //                solid angles.  HBM-bound: 16 B per texel streamed once.
