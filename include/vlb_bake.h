@@ -12,7 +12,8 @@
  * Conventions
  *  - Every function returns vlb_status (0 == VLB_OK); nothing throws across
  *    the ABI.  vlb_last_error(ctx) returns a message for the last failure on
- *    that context (thread-local to the context, valid until the next call).
+ *    that context, from any thread; the string is the calling thread's copy,
+ *    valid until that thread's next vlb_last_error call.
  *  - Host pointers unless the name says "device" / "_d".  The caller owns all
  *    buffers it passes in.
  *  - Coefficient layout everywhere: out[probe][coef][rgb], float32, with

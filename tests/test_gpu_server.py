@@ -6,6 +6,8 @@ whatever else is in flight, the step size or the admission budget; FP32
 requests are within 1e-5 relative L2 of the FP64 oracle, BF16 (split) within
 1e-4.
 """
+import threading
+
 import numpy as np
 import pytest
 
@@ -153,3 +155,38 @@ def test_server_rejects_bad_requests_and_drains_on_destroy(ctx, scenes):
     assert info["status"] == 0
     srv.submit(sid, _params(_probes(20, 3)))   # pending at destroy: cancelled
     srv.destroy()
+
+
+def test_concurrent_failing_submits_report_their_errors(ctx, scenes):
+    # requests fail from several threads at once; each reads a whole message
+    (_, sid) = scenes[0]
+    srv = vlb.Server(ctx, precision=vlb.PREC_FP32, max_lights=1)
+    two = [(vlb.LIGHT_DIRECTIONAL, (0, -1, 0), (1, 1, 1))] * 2
+    bad = [_params(_probes(8, 1), precision=vlb.PREC_BF16), _params(_probes(8, 2), lights=two)]
+    words = ["precision", "max_lights"]
+    seen, errors = [], []
+
+    def worker(k):
+        try:
+            for i in range(200):
+                j = (i + k) % 2
+                try:
+                    srv.submit(sid, bad[j])
+                    errors.append("accepted a bad request")
+                except vlb.VlbError as e:
+                    assert e.status == vlb.ERR_INVALID
+                    msg = str(e)
+                    seen.append(msg)
+                    if not any(w in msg for w in words):
+                        errors.append(msg)
+        except Exception as e:  # noqa: BLE001 -- reported below
+            errors.append(repr(e))
+
+    ths = [threading.Thread(target=worker, args=(k,)) for k in range(8)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    srv.destroy()
+    assert not errors, errors[:3]
+    assert len(seen) == 1600

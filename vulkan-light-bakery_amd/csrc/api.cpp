@@ -433,8 +433,8 @@ vlb_status vlb_ctx_destroy(vlb_ctx* h)
     {
         std::lock_guard<std::recursive_mutex> lk(c->mtx);
         if (c->children > 0) {
-            c->last_error = "context still has " + std::to_string(c->children) +
-                            " live plans, batches or servers: destroy them first";
+            set_error(c, "context still has " + std::to_string(c->children) +
+                             " live plans, batches or servers: destroy them first");
             return VLB_ERR_INVALID;
         }
     }
@@ -462,7 +462,11 @@ vlb_status vlb_debug_inject(vlb_ctx* h, int32_t site, int32_t count)
 const char* vlb_last_error(const vlb_ctx* h)
 {
     if (!h) return "null context";
-    return reinterpret_cast<const Context*>(h)->last_error.c_str();
+    // a copy per calling thread: another thread's failure may replace the
+    // context's message at any time
+    thread_local std::string msg;
+    msg = get_error(reinterpret_cast<const Context*>(h));
+    return msg.c_str();
 }
 
 vlb_status vlb_device_info(vlb_ctx* h, int32_t* num_cus, char* name, size_t name_len)
@@ -1027,7 +1031,7 @@ vlb_status vlb_skybox_sh(vlb_ctx* h, const float* cubes, uint32_t face_size, uin
         DevBuf d_out = dev_alloc(*c, n_out * sizeof(float));
         VLB_HIP(hipMemcpyAsync(d_in.p, cubes, in_bytes, hipMemcpyHostToDevice, c->stream));
         vlb_status s = vlb_skybox_sh_d(h, d_in.p, face_size, batch, sh_order, d_out.p);
-        if (s != VLB_OK) throw Error{s, c->last_error};
+        if (s != VLB_OK) throw Error{s, get_error(c)};
         VLB_HIP(hipMemcpyAsync(out, d_out.p, n_out * sizeof(float), hipMemcpyDeviceToHost, c->stream));
         VLB_HIP(hipStreamSynchronize(c->stream));
     });

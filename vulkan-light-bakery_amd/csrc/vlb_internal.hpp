@@ -119,6 +119,10 @@ struct Context {
     hipStream_t stream = nullptr;
     uint64_t budget = 0;                  // bytes
     std::atomic<uint64_t> in_use{0};      // bytes allocated through dev_alloc
+    // message of the last failure on this context, from any thread (the
+    // serving entry points may fail concurrently): written and copied out
+    // under err_mu only (set_error / get_error)
+    mutable std::mutex err_mu;
     std::string last_error;
     std::map<uint32_t, std::unique_ptr<Scene>> scenes;
     uint32_t next_scene = 1;
@@ -183,6 +187,18 @@ DevBuf upload_vec(Context& ctx, const std::vector<T>& v)
     return b;
 }
 
+inline void set_error(Context* ctx, const std::string& msg)
+{
+    std::lock_guard<std::mutex> lk(ctx->err_mu);
+    ctx->last_error = msg;
+}
+
+inline std::string get_error(const Context* ctx)
+{
+    std::lock_guard<std::mutex> lk(ctx->err_mu);
+    return ctx->last_error;
+}
+
 // Runs f, turning exceptions into a status code and the context's last error.
 template <class F>
 inline vlb_status guard(Context* ctx, F&& f)
@@ -191,16 +207,16 @@ inline vlb_status guard(Context* ctx, F&& f)
         f();
         return VLB_OK;
     } catch (const Error& e) {
-        if (ctx) ctx->last_error = e.msg;
+        if (ctx) set_error(ctx, e.msg);
         return e.code;
     } catch (const std::bad_alloc&) {
-        if (ctx) ctx->last_error = "host allocation failed";
+        if (ctx) set_error(ctx, "host allocation failed");
         return VLB_ERR_OOM;
     } catch (const std::exception& e) {
-        if (ctx) ctx->last_error = e.what();
+        if (ctx) set_error(ctx, e.what());
         return VLB_ERR_INVALID;
     } catch (...) {
-        if (ctx) ctx->last_error = "unknown error";
+        if (ctx) set_error(ctx, "unknown error");
         return VLB_ERR_INVALID;
     }
 }
