@@ -567,7 +567,7 @@ def bench_c2(args):
                           cnt["tri_tests"], st["trace_ms"] / 1e3, 1, load_traffic("bake_c2_traffic.json"))
 
     orc = oracle_leg(args)
-    err_line, cpu = None, None
+    err_line, cpu, hits = None, None, None
     if orc is not None:
         rng = np.random.default_rng(0)
         sub = np.sort(rng.choice(P, size=64, replace=False))
@@ -577,6 +577,9 @@ def bench_c2(args):
         ref = oscene.bake(psub, mode64=True, nthreads=CPU_THREADS)["coeffs"]
         err = rel_err(coeffs[sub], ref)
         err_line = {"max": float(err.max()), "p99": float(np.percentile(err, 99)), "probes": len(sub)}
+        # hit parity over every primary ray of the bake
+        hits = _hit_parity(ctx, vlb, [scene], [sid], lambda i: oscene,
+                           [(0, vlb.config_c2_params(scene, seed=2))], variant)
         # the FP32 C oracle on the same scene, probes and rays: whole c2
         # bakes repeated until >= cpu_seconds of wall time on the host's cores
         pc = vlb.config_c2_params(scene, seed=2)
@@ -617,6 +620,7 @@ def bench_c2(args):
         "kernel_ms": {"trace": round(st["trace_ms"], 4), "project": round(st["project_ms"], 4),
                       "merge": round(st["merge_ms"], 4), "timed_runs": st["timed_runs"]},
         "sh_rel_err_vs_fp64": err_line,
+        "hit_parity": hits,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -828,6 +832,9 @@ def bench_c3(args):
                 band[f"l{l}"] = float(rel_err(got[:, ks], ref[:, ks]).max())
             band["all"] = float(rel_err(got, ref).max())
             errs[f"L{order}_{name}"] = band
+    # hit parity over every primary ray of 8 of the seeded probes
+    hits = (_hit_parity(ctx, vlb, [scene], [sid], lambda i: osc, [(0, params(0, 2, probes[sub[:8]].copy()))],
+                        variant) if osc is not None else None)
     rays = cnt["primary_rays"] + cnt["shadow_rays"]
     tr = runs["bf16"]["trace_ms"] / 1e3
     line = {"metric": METRIC, "value": runs["bf16"]["probes_per_s"], "unit": "probes/s", "n_gpus": 1,
@@ -841,6 +848,7 @@ def bench_c3(args):
             "per_ray": {"inner_nodes": round(cnt["node_visits"] / rays, 2),
                         "triangles": round(cnt["tri_tests"] / rays, 2)},
             "sh_rel_err_vs_fp64_by_band": errs or None, "error_probes": len(sub) if errs else 0,
+            "hit_parity": hits,
             "cpu_baseline": cpu,
             "roofline": trace_roofline("k_trace" if variant <= 12 else "k_rays + k_shade", variant,
                                        cnt["node_visits"], cnt["tri_tests"], tr, 1,
