@@ -482,9 +482,8 @@ def bench_c5(args):
     sids = [ctx.upload_scene_dict(s) for s in host]
     infos = [ctx.scene_info(s) for s in sids]
     res, reqs, ps16, ps32, outs = serve(ctx, vlb, W, host, sids, args)
-    ctx.close()
     orc = oracle_leg(args)
-    err = cpu = None
+    err = cpu = hits = None
     if orc is not None:
         oscenes = {}
 
@@ -495,6 +494,17 @@ def bench_c5(args):
 
         n = len(reqs)
         err = _err_sample(vlb, oracle_of, [(reqs[i][0], ps16[i], outs[i]) for i in range(0, n, max(1, n // 20))])
+        # hit parity: 2 seeded probes of the first request on each scene
+        rng = np.random.default_rng(9)
+        picks, seen = [], set()
+        for i, (si, pr, _, _) in enumerate(reqs):
+            if si in seen:
+                continue
+            seen.add(si)
+            q = vlb.Params(**{**ps32[i].__dict__, "_keep": []})
+            q.probes = pr[np.sort(rng.choice(pr.shape[0], size=2, replace=False))].copy()
+            picks.append((si, q))
+        hits = _hit_parity(ctx, vlb, host, sids, oracle_of, picks, args.variant or 12)
         spp_scene, t_cpu = {}, 0.0
         for i in range(n):
             si = reqs[i][0]
@@ -507,6 +517,7 @@ def bench_c5(args):
         cpu = cpu_line(round(res["probes"] / est, 2), "probes/s",
                        f"FP32 C oracle, 16 probes of one request per scene ({len(spp_scene)} scenes) baked "
                        f"repeatedly and scaled to all {n} requests, {CPU_THREADS} threads, {t_cpu:.1f} s")
+    ctx.close()
     lat = res["latency_ms"]
     return {"metric": METRIC, "value": res["probes_per_s"], "unit": "probes/s", "n_gpus": 1,
             "steps": res["requests"], "warmup": len(host), "ms_per_step": lat["p50"],
@@ -518,7 +529,7 @@ def bench_c5(args):
                                    "bvh_depth": infos[i]["depth"]} for i in range(len(host))]},
             "latency_ms": lat, "server": res["server"],
             "plain_fp32_per_request": res["plain_fp32_per_request"],
-            "sh_rel_err_vs_fp64": err, "cpu_baseline": cpu}
+            "sh_rel_err_vs_fp64": err, "hit_parity": hits, "cpu_baseline": cpu}
 
 
 # -------------------------------------------------------- plain single bake
