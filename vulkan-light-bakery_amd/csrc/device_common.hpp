@@ -459,22 +459,29 @@ __device__ __forceinline__ void cswap(float& ea, int& ca, float& eb, int& cb)
 
 // After the leaf tests (their e set to +inf): sort the inner children near
 // first, push all hit ones but the nearest (farthest first) and continue at
-// the nearest, else pop.  Returns false when the stack is empty.
+// the nearest, else pop.  Returns false when the stack is empty.  SORT =
+// false (shadow queries, which stop at any hit) keeps the children in slot
+// order: the sort's 20 instructions per visit buy no fewer visits there
+// (0.1-0.3% fewer without it on the c2-c4 scenes), and the pushes below are
+// right for any order (a missed child is written and then overwritten).
+template <bool SORT = true>
 __device__ __forceinline__ bool next_node(float e[4], int c[4], const Stack& st, int& sp,
                                           uint32_t& node)
 {
-    cswap(e[0], c[0], e[1], c[1]);
-    cswap(e[2], c[2], e[3], c[3]);
-    cswap(e[0], c[0], e[2], c[2]);
-    cswap(e[1], c[1], e[3], c[3]);
-    cswap(e[1], c[1], e[2], c[2]);
+    if (SORT) {
+        cswap(e[0], c[0], e[1], c[1]);
+        cswap(e[2], c[2], e[3], c[3]);
+        cswap(e[0], c[0], e[2], c[2]);
+        cswap(e[1], c[1], e[3], c[3]);
+        cswap(e[1], c[1], e[2], c[2]);
+    }
     const float inf = __builtin_huge_valf();
     if (__all(sp + 3 <= kLdsDepth)) {
         // Every active lane has three free LDS entries (one wave-uniform
         // test): straight-line pushes with conditional increments.  A write
         // past the top lands at most where the all-hit case writes, and is
-        // overwritten by the next push or left unused.  Hit entries are
-        // sorted first, so the pushes are the same as below.
+        // overwritten by the next push or left unused, so the pushes are the
+        // same as below in any order.
         st.lds[sp * 256] = (uint32_t)c[3];
         sp += e[3] != inf ? 1 : 0;
         st.lds[sp * 256] = (uint32_t)c[2];
@@ -573,7 +580,7 @@ __device__ bool trace_any(const SceneView& sc, const float3& o, const float3& d,
             WtHit h;
             if (wt_test(w, v0, v1, v2, tmin, tmax, h)) return true;
         }
-        if (!next_node(e, c, st, sp, node)) break;
+        if (!next_node<false>(e, c, st, sp, node)) break;
     }
     return false;
 }

@@ -154,7 +154,7 @@ __device__ __forceinline__ void trav_init(TravState& T, float maxabs, const floa
 // straight-line writes with conditional increments; needs nq <= kLeafQueue
 // - 4, so a write past the end stays inside the queue), hit inner children
 // go on the stack.
-template <bool COUNT>
+template <bool ANY, bool COUNT>
 __device__ __forceinline__ void visit_node(TravState& T, const QNode4* nodes, const Stack& st,
                                            const LeafQueue& q, TraceCount& cnt)
 {
@@ -169,7 +169,7 @@ __device__ __forceinline__ void visit_node(TravState& T, const QNode4* nodes, co
         q.lds[T.nq * 256] = (uint32_t)~c[k];
         T.nq += (int)((leaves >> k) & 1u);
     }
-    T.live = next_node(e, c, st, T.sp, T.node);
+    T.live = next_node<!ANY>(e, c, st, T.sp, T.node);  // shadow queries: slot order
 }
 
 // Tests every queued triangle of the active lanes (the wave runs as many
@@ -209,7 +209,7 @@ template <bool ANY, bool COUNT>
 __device__ __forceinline__ bool trav_step(TravState& T, const QNode4* nodes, const float4* tris,
                                           const Stack& st, const LeafQueue& q, TraceCount& cnt)
 {
-    if (T.live && T.nq <= kLeafQueue - 4) visit_node<COUNT>(T, nodes, st, q, cnt);
+    if (T.live && T.nq <= kLeafQueue - 4) visit_node<ANY, COUNT>(T, nodes, st, q, cnt);
     const bool ready = T.nq > 0 || !T.live;
     if ((__all(ready) && __any(T.nq > 0)) || __any(T.nq > kLeafQueue - 4))
         test_queue<ANY, COUNT>(T, tris, q, cnt);
