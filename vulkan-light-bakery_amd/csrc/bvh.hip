@@ -462,9 +462,18 @@ __global__ void k_collapse(const BNode* __restrict__ n2, const int* __restrict__
         }
     }
     const int used = m;
-    for (; m < 4; ++m) {                 // never tested (slab4)
-        for (int t = 0; t < 6; ++t) bx[m][t] = 0.0f;
+    // Unused slots: in the float node an all-+inf box, which the slab test
+    // misses for every ray (each axis gives t = +inf or -inf, or NaN, which
+    // min/max drop, and a unit direction keeps one axis finite), with leaf 0
+    // as its id, so that even a box it did not miss would only re-test a
+    // real triangle; in the quantized node kEmpty (its decoded planes are
+    // finite, slab4<true> checks the id).
+    int chf[4];
+    for (int k = 0; k < used; ++k) chf[k] = ch[k];
+    for (; m < 4; ++m) {
+        for (int t = 0; t < 6; ++t) bx[m][t] = __builtin_huge_valf();
         ch[m] = kEmpty;
+        chf[m] = ~0;
     }
     BNode4 o;
     o.lox = make_float4(bx[0][0], bx[1][0], bx[2][0], bx[3][0]);
@@ -473,7 +482,7 @@ __global__ void k_collapse(const BNode* __restrict__ n2, const int* __restrict__
     o.hix = make_float4(bx[0][3], bx[1][3], bx[2][3], bx[3][3]);
     o.hiy = make_float4(bx[0][4], bx[1][4], bx[2][4], bx[3][4]);
     o.hiz = make_float4(bx[0][5], bx[1][5], bx[2][5], bx[3][5]);
-    o.child = make_int4(ch[0], ch[1], ch[2], ch[3]);
+    o.child = make_int4(chf[0], chf[1], chf[2], chf[3]);
     o.pad = make_int4(0, 0, 0, 0);
     n4[idx4[i]] = o;
     float org[3], stp[3];

@@ -404,6 +404,9 @@ __device__ __forceinline__ uint32_t take_leaves(float e[4], const int c[4])
 }
 
 // Entry distances of the four children (+inf: missed or empty slot).
+// CHECK_EMPTY: the node's unused slots are marked kEmpty (quantized nodes);
+// a float node's unused slots hold all-+inf boxes that the slab test misses.
+template <bool CHECK_EMPTY>
 __device__ __forceinline__ void slab4(const SlabRay& s, const BNode4& n, float tmin, float tbest,
                                       float e[4], int c[4])
 {
@@ -415,9 +418,11 @@ __device__ __forceinline__ void slab4(const SlabRay& s, const BNode4& n, float t
     e[1] = slab(s, n.lox.y, n.loy.y, n.loz.y, n.hix.y, n.hiy.y, n.hiz.y, tmin, tbest);
     e[2] = slab(s, n.lox.z, n.loy.z, n.loz.z, n.hix.z, n.hiy.z, n.hiz.z, tmin, tbest);
     e[3] = slab(s, n.lox.w, n.loy.w, n.loz.w, n.hix.w, n.hiy.w, n.hiz.w, tmin, tbest);
+    if (CHECK_EMPTY) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (c[k] == kEmpty) e[k] = __builtin_huge_valf();
+        for (int k = 0; k < 4; ++k)
+            if (c[k] == kEmpty) e[k] = __builtin_huge_valf();
+    }
 }
 
 // Entry distances and ids of the children of node i, and its hit leaf
@@ -438,11 +443,11 @@ __device__ __forceinline__ uint32_t fetch_slab4(const NodeT* nodes, uint32_t i, 
         (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
     if (__all(a == u)) {
         const BNode4 n = load_node_uniform(reinterpret_cast<const NodeT*>(u));
-        slab4(s, n, tmin, tbest, e, c);
+        slab4<std::is_same<NodeT, QNode4>::value>(s, n, tmin, tbest, e, c);
         return take_leaves(e, c);
     }
     const BNode4 n = load_node(nodes, i);
-    slab4(s, n, tmin, tbest, e, c);
+    slab4<std::is_same<NodeT, QNode4>::value>(s, n, tmin, tbest, e, c);
     return take_leaves(e, c);
 }
 

@@ -162,7 +162,7 @@ __device__ __forceinline__ void visit_node(TravState& T, const QNode4* nodes, co
     if (COUNT) ++cnt.nodes;
     float e[4];
     int c[4];
-    slab4(T.s, n, T.tmin, T.tb, e, c);
+    slab4<true>(T.s, n, T.tmin, T.tb, e, c);
     const uint32_t leaves = take_leaves(e, c);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
