@@ -47,11 +47,16 @@ HBM_PEAK_GBS = 8000.0
 TRI_BYTES = 48.0     # bytes per triangle test (3 x float4)
 
 
+def is_fused(variant):
+    """trace_variant 1..12 and 25..36 (far-first shadow queries) run the fused
+    trace+shade kernel, 13..24 the wavefront kernels."""
+    return not 13 <= variant <= 24
+
+
 def node_bytes(variant):
-    """Bytes fetched per BVH4 node visit: the fused kernels (trace_variant
-    1..12) read float nodes (7 x 16 B in use), the wavefront kernels
-    (13..24) quantized ones (4 x 16 B)."""
-    return 112.0 if variant <= 12 else 64.0
+    """Bytes fetched per BVH4 node visit: the fused kernels read float nodes
+    (7 x 16 B in use), the wavefront kernels quantized ones (4 x 16 B)."""
+    return 112.0 if is_fused(variant) else 64.0
 
 
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16_fast": 3, "fp16_fast": 4}
@@ -430,7 +435,7 @@ def bench_c4(args):
 
     rays = cnt["primary_rays"] + cnt["shadow_rays"]
     rounds = info["rounds"]
-    roof = trace_roofline("k_fused_jobs" if variant <= 12 else "k_rays_jobs + k_shade_jobs", variant,
+    roof = trace_roofline("k_fused_jobs" if is_fused(variant) else "k_rays_jobs + k_shade_jobs", variant,
                           cnt["node_visits"], cnt["tri_tests"], info["trace_ms"] / 1e3 / rounds, rounds,
                           load_traffic("batch_c4_traffic.json"))
     return {
@@ -574,7 +579,7 @@ def bench_c2(args):
     probes_per_s = P * args.steps / elapsed
     rays_per_step = cnt["primary_rays"] + cnt["shadow_rays"]
     grays = rays_per_step * args.steps / elapsed / 1e9
-    roof = trace_roofline("k_trace" if variant <= 12 else "k_rays + k_shade", variant, cnt["node_visits"],
+    roof = trace_roofline("k_trace" if is_fused(variant) else "k_rays + k_shade", variant, cnt["node_visits"],
                           cnt["tri_tests"], st["trace_ms"] / 1e3, 1, load_traffic("bake_c2_traffic.json"))
 
     orc = oracle_leg(args)
@@ -748,7 +753,7 @@ def sweep_c2(args):
     ctx = vlb.Context(0)
     sid = ctx.upload_scene_dict(scene)
     rows = []
-    for v in range(1, 25):
+    for v in range(1, 37):
         params = vlb.config_c2_params(scene, seed=2)
         params.trace_variant = v
         plan = ctx.plan(sid, params)
@@ -759,9 +764,10 @@ def sweep_c2(args):
             plan.execute()
         plan.sync()
         st = plan.stats()
-        kind = "wave" if v >= 13 else "fused"
-        rows.append({"variant": v, "kind": kind, "map": ((v - 13) if v >= 13 else (v - 1)) % 4,
-                     "occ_or_refill": ([16, 32, 8][(v - 13) // 4] if v >= 13 else [0, 6, 8][(v - 1) // 4]),
+        kind = "fused" if v <= 12 else ("wave" if v <= 24 else "fused far-first")
+        b = (v - 1) % 12
+        rows.append({"variant": v, "kind": kind, "map": b % 4,
+                     "occ_or_refill": ([16, 32, 8][b // 4] if kind == "wave" else [0, 6, 8][b // 4]),
                      "trace_ms": round(st["trace_ms"], 4), "project_ms": round(st["project_ms"], 4),
                      "total_ms": round(st["total_ms"], 4), "grid": st["grid"]})
         print(json.dumps(rows[-1]), flush=True)
@@ -861,7 +867,7 @@ def bench_c3(args):
             "sh_rel_err_vs_fp64_by_band": errs or None, "error_probes": len(sub) if errs else 0,
             "hit_parity": hits,
             "cpu_baseline": cpu,
-            "roofline": trace_roofline("k_trace" if variant <= 12 else "k_rays + k_shade", variant,
+            "roofline": trace_roofline("k_trace" if is_fused(variant) else "k_rays + k_shade", variant,
                                        cnt["node_visits"], cnt["tri_tests"], tr, 1,
                                        load_traffic("bake_c3_traffic.json"))}
     ctx.close()

@@ -180,8 +180,10 @@ typedef struct vlb_bake_params {
                                       /* 13..24 wavefront kernels (closest,  */
                                       /*   shadow, shade): 13 + map +        */
                                       /*   4*refill, refill {16, 32, 8}      */
-                                      /*   idle lanes; 0 = autotuned choice, */
-                                      /*   else 12                           */
+                                      /*   idle lanes; 25..36 as 1..12 with  */
+                                      /*   shadow rays visiting BVH children */
+                                      /*   far first (same results); 0 =     */
+                                      /*   autotuned choice, else 12         */
     uint32_t         reserved[2];
 } vlb_bake_params;
 
@@ -296,7 +298,8 @@ vlb_status  vlb_plan_use_graph(vlb_plan* plan, int32_t on);
  * tests, shadow rays) and fills the counters of vlb_bake_stats. */
 vlb_status  vlb_plan_count(vlb_plan* plan);
 /* Autotuner: times the trace-kernel variants (lane->ray maps x occupancy
- * bounds) on this plan's real workload, `iters` runs each, keeps the fastest
+ * bounds x shadow order, and the wavefront kernels) on this plan's real
+ * workload, `iters` runs each, keeps the fastest
  * for the plan and remembers it for later plans on the same scene and ray
  * count.  *chosen (may be NULL) receives the trace_variant picked. */
 vlb_status  vlb_plan_autotune(vlb_plan* plan, uint32_t iters, uint32_t* chosen);
@@ -333,7 +336,8 @@ typedef struct vlb_batch_options {
     uint64_t max_round_rays;  /* 0 = 2^28; at most 2^30                   */
     uint32_t trace_variant;   /* 1..12 fused kernel (map x waves/SIMD as */
                               /* in vlb_bake_params), 13..24 wavefront  */
-                              /* lane map x refill; 0 = 12               */
+                              /* lane map x refill, 25..36 fused with   */
+                              /* far-first shadow rays; 0 = 12          */
     uint32_t chunk;           /* rays per work fetch: 64..1024, divides   */
                               /* 1024; 0 = 256                            */
     uint32_t keep_f64;        /* also keep FP64 coefficients              */
@@ -374,7 +378,8 @@ vlb_status  vlb_batch_download(vlb_batch* b, uint32_t job, float* out, size_t ou
 vlb_status  vlb_batch_download_f64(vlb_batch* b, uint32_t job, double* out, size_t out_len);
 /* Autotuner: times the wavefront lane maps x refill thresholds
  * (trace_variant 13..24) x work-fetch chunks {256, 1024} and the fused
- * kernel's lane maps 2, 3 at 6 and 8 waves/SIMD (7, 8, 11, 12) on a sample of
+ * kernel's lane maps 2, 3 at 6 and 8 waves/SIMD with either shadow order
+ * (7, 8, 11, 12, 31, 32, 35, 36) on a sample of
  * every job's tiles (about 16M primary rays), `iters` runs each, and keeps
  * the fastest. */
 vlb_status  vlb_batch_autotune(vlb_batch* b, uint32_t iters, uint32_t* variant,
@@ -407,7 +412,7 @@ typedef struct vlb_server_options {
     uint32_t max_active;      /* requests on the device at once; 0 = 64   */
     uint32_t max_lights;      /* shadow rays per primary ray; 0 = 4       */
     int32_t  precision;       /* VLB_PREC_* of every request              */
-    uint32_t trace_variant;   /* 1..24 as in vlb_batch_options; 0 = 12    */
+    uint32_t trace_variant;   /* 1..36 as in vlb_batch_options; 0 = 12    */
     uint32_t chunk;           /* rays per work fetch; 0 = 256             */
     uint32_t max_depth;       /* deepest scene BVH served; 0 = 64         */
     uint32_t policy;          /* VLB_SCHED_*: how a step's rays are shared */

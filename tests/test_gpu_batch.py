@@ -70,8 +70,9 @@ def test_batch_jobs_equal_single_bakes_bitwise(ctx, scenes):
     single = [ctx.bake(sid, p) for sid, p in jobs]
     # small rounds: jobs split across rounds, rounds hold several jobs
     # wavefront and fused kernels
+    # (25..36: fused with far-first shadow queries)
     for mrr, var, chunk in [(0, 20, 256), (1 << 16, 13, 1024), (1 << 14, 18, 64), (0, 12, 256),
-                            (1 << 16, 1, 256), (1 << 14, 7, 256)]:
+                            (1 << 16, 1, 256), (1 << 14, 7, 256), (0, 36, 256), (1 << 16, 25, 256)]:
         b = ctx.batch(jobs, max_round_rays=mrr, trace_variant=var, chunk=chunk)
         info = b.info()
         if mrr:
@@ -145,8 +146,8 @@ def test_batch_autotune_and_counts(ctx, scenes):
     b.execute()
     before = [b.download(j) for j in range(len(jobs))]
     v, c = b.autotune(1)
-    # fused kernel (1..12) at chunk 256 or wavefront kernels (13..24)
-    assert (1 <= v <= 12 and c == 256) or (13 <= v <= 24 and c in (256, 1024))
+    # fused kernel (1..12, 25..36) at chunk 256 or wavefront kernels (13..24)
+    assert ((1 <= v <= 12 or 25 <= v <= 36) and c == 256) or (13 <= v <= 24 and c in (256, 1024))
     b.execute()
     for j in range(len(jobs)):
         assert np.array_equal(_bits(b.download(j)), _bits(before[j]))
@@ -185,7 +186,7 @@ def test_batch_rejects_mixed_precision_and_bad_options(ctx, scenes):
     with pytest.raises(vlb.VlbError):
         ctx.batch([a, b])
     with pytest.raises(vlb.VlbError):
-        ctx.batch([a], trace_variant=25)
+        ctx.batch([a], trace_variant=37)
     with pytest.raises(vlb.VlbError):
         ctx.batch([a], chunk=100)
 

@@ -288,9 +288,10 @@ def test_plan_graph_replays_match_eager(ctx, room):
     pl.destroy()
 
 
-# ---- every trace pipeline computes the same bits: fused k_trace (1..12) and
-# the wavefront k_rays/k_shade pipeline (13..24)
-VARIANTS = [1, 3, 4, 8, 12, 13, 15, 16, 20, 23]
+# ---- every trace pipeline computes the same bits: fused k_trace (1..12), the
+# wavefront k_rays/k_shade pipeline (13..24) and fused with far-first shadow
+# queries (25..36)
+VARIANTS = [1, 3, 4, 8, 12, 13, 15, 16, 20, 23, 25, 28, 32, 36]
 
 
 def test_trace_variants_are_bitwise_identical(ctx, room):
@@ -304,6 +305,38 @@ def test_trace_variants_are_bitwise_identical(ctx, room):
             ref = g
             assert rel_err(g, o.bake(p, mode64=True)["coeffs"]).max() <= 1e-5
         assert np.array_equal(g.view(np.uint32), ref.view(np.uint32)), v
+
+
+@pytest.mark.parametrize("mode", [vlb.SHADE_LAMBERT, vlb.SHADE_REFERENCE])
+def test_far_first_shadows_same_bits_and_shadow_rays(ctx, room, mode):
+    """Shadow queries visiting children far first (25..36) answer as the
+    slot-order ones (1..12): the same coefficients bit for bit and the same
+    shadow rays, with a different node-visit count from the counting pass."""
+    s, sid, o = room
+    p = _bake_params(_probes(33, seed=5), rays=500, shade_mode=mode, ref_light=(5.0, 3.5, 5.0))
+    got = {}
+    for v in (12, 36, 25):
+        p.trace_variant = v
+        pl = ctx.plan(sid, p)
+        pl.execute()
+        g = pl.download()
+        pl.count()
+        got[v] = (g, pl.stats())
+        pl.destroy()
+    assert rel_err(got[12][0], o.bake(p, mode64=True)["coeffs"]).max() <= 1e-5
+    for v in (36, 25):
+        assert np.array_equal(got[v][0].view(np.uint32), got[12][0].view(np.uint32)), v
+        assert got[v][1]["shadow_rays"] == got[12][1]["shadow_rays"] > 0
+        assert got[v][1]["primary_rays"] == got[12][1]["primary_rays"]
+    assert got[36][1]["node_visits"] == got[25][1]["node_visits"]
+    assert got[36][1]["node_visits"] != got[12][1]["node_visits"]
+
+
+def test_trace_variant_range(ctx, room):
+    s, sid, o = room
+    for v in (37, 100):
+        with pytest.raises(vlb.VlbError):
+            ctx.bake(sid, _bake_params(_probes(4), rays=64, trace_variant=v))
 
 
 @pytest.mark.parametrize("variant", [15, 16])
@@ -379,7 +412,7 @@ def test_autotune_picks_a_valid_variant_and_keeps_results(ctx, room):
     pl.execute()
     before = pl.download()
     v = pl.autotune(1)
-    assert 1 <= v <= 24
+    assert 1 <= v <= 36
     pl.execute()
     after = pl.download()
     assert np.array_equal(before.view(np.uint32), after.view(np.uint32))

@@ -67,8 +67,9 @@ def _bits(a):
 
 # 0: the server's default (the fused kernel, variant 12); 12 and 3: the
 # fused kernel at 8 waves/SIMD and at the compiler's occupancy; 20: the
-# wavefront multi-job kernels (k_rays_jobs / k_shade_jobs, lane map 3)
-@pytest.mark.parametrize("variant", [0, 12, 3, 20])
+# wavefront multi-job kernels (k_rays_jobs / k_shade_jobs, lane map 3); 36:
+# variant 12 with far-first shadow queries
+@pytest.mark.parametrize("variant", [0, 12, 3, 20, 36])
 def test_server_request_equals_single_bake(ctx, scenes, variant):
     reqs = _requests(scenes, n=3)
     srv = vlb.Server(ctx, step_rays=1 << 16, trace_variant=variant)

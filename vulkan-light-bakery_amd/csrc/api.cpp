@@ -79,7 +79,8 @@ static void drain_timing(Plan& pl)
 namespace {
 
 // trace_variant: 1..12 fused k_trace (1 + map + 4 * occupancy index),
-// 13..24 wavefront k_rays/k_shade (13 + map + 4 * refill index)
+// 13..24 wavefront k_rays/k_shade (13 + map + 4 * refill index), 25..36 as
+// 1..12 with far-first shadow queries
 constexpr uint32_t kFusedDefault = 12;  // fused, lane map 3, 8 waves/SIMD
 
 // trace_variant of a plan that leaves it to the library: the fused kernel,
@@ -92,17 +93,19 @@ uint32_t default_variant(bool /*indirect*/)
 {
     return kFusedDefault;
 }
-constexpr uint32_t kMaxVariant = 24;
+constexpr uint32_t kMaxVariant = 36;
 constexpr int kOccTab[3] = {0, 6, 8};
 constexpr int kRefillTab[3] = {16, 32, 8};
 
 struct Variant {
-    bool wave;
+    bool wave, far;
     int map, occ, refill;
 };
 Variant decode_variant(uint32_t v)
 {
     Variant d{};
+    d.far = v >= 25;
+    if (d.far) v -= 24;
     if (v >= 13) {
         d.wave = true;
         d.map = (int)((v - 13) % 4);
@@ -133,6 +136,7 @@ void apply_variant(Context& ctx, Plan& pl, uint32_t v)
     const Variant d = decode_variant(v);
     pl.variant = v;
     pl.wave = d.wave;
+    pl.far = d.far;
     pl.occ = d.occ;
     pl.args.map = d.map;
     pl.args.refill = d.refill;
@@ -140,7 +144,7 @@ void apply_variant(Context& ctx, Plan& pl, uint32_t v)
         ensure_wave_buffers(ctx, pl);
         pl.args.spill_stride = (uint32_t)pl.wgrid * 256u;
     } else {
-        pl.grid = trace_grid_size(ctx, d.occ);
+        pl.grid = trace_grid_size(ctx, d.occ, d.far);
         pl.args.spill_stride = (uint32_t)pl.grid * 256u;
     }
     pl.stats.grid = (uint32_t)(d.wave ? pl.wgrid : pl.grid);
@@ -156,7 +160,7 @@ void apply_variant(Context& ctx, Plan& pl, uint32_t v)
 void launch_trace_stage(Plan& pl, bool record, bool count, hipStream_t st)
 {
     if (pl.wave) launch_wave(pl.args, record, count, pl.wgrid, st);
-    else launch_trace(pl.args, record, count, record || count ? 0 : pl.occ, pl.grid, st);
+    else launch_trace(pl.args, record, count, record || count ? 0 : pl.occ, pl.far, pl.grid, st);
 }
 
 
@@ -185,9 +189,10 @@ void build_plan(Context& ctx, Scene& sc, const vlb_bake_params& p, Plan& pl)
         tv = pl.default_variant ? default_variant(false) : it->second;
     }
     VLB_REQUIRE(tv >= 1 && tv <= kMaxVariant, VLB_ERR_INVALID, "trace_variant out of range");
-    pl.grid = trace_grid_size(ctx, 0);
+    pl.grid = trace_grid_size(ctx, 0, false);
     pl.wgrid = wave_grid_size(ctx);
-    const int max_grid = std::max(std::max(pl.grid, pl.wgrid), trace_grid_size(ctx, 8));
+    // 8 waves/SIMD is the most any trace kernel runs at
+    const int max_grid = std::max(std::max(pl.grid, pl.wgrid), trace_grid_size(ctx, 8, false));
     pl.pgrid = ctx.num_cus * 8;
     const int npg = (int)((P + 15) / 16);
 
@@ -803,9 +808,11 @@ vlb_status vlb_plan_autotune(vlb_plan* h, uint32_t iters, uint32_t* chosen)
         hipEvent_t e0, e1;
         VLB_HIP(hipEventCreate(&e0));
         VLB_HIP(hipEventCreate(&e1));
-        // fused: lane maps 0..3 unbounded + maps 2, 3 at 6 and 8 waves/SIMD;
-        // wavefront: maps 0..3 with refill 16, maps 2, 3 with refill 32 / 8
-        const uint32_t cand[] = {1, 2, 3, 4, 7, 8, 11, 12, 13, 14, 15, 16, 19, 20, 23, 24};
+        // fused: lane maps 0..3 unbounded + maps 2, 3 at 6 and 8 waves/SIMD,
+        // each with slot-order and far-first shadows; wavefront: maps 0..3
+        // with refill 16, maps 2, 3 with refill 32 / 8
+        const uint32_t cand[] = {1,  2,  3,  4,  7,  8,  11, 12, 25, 26, 27, 28,
+                                 31, 32, 35, 36, 13, 14, 15, 16, 19, 20, 23, 24};
         double best = 1e30;
         uint32_t best_v = pl->variant;
         for (uint32_t v : cand) {

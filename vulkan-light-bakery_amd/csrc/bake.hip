@@ -46,8 +46,9 @@ __global__ void k_zero_ctr8(unsigned long long* c)
 // (tile_map, a.map) sets how many neighbouring probes x directions (probes
 // Morton sorted, directions on a Hilbert curve) one wave covers.
 // OCC > 0 asks the compiler for OCC waves per SIMD (VGPR budget 512/OCC).
-// IND: plans with the multi-bounce term.
-template <bool RECORD, bool COUNT, int OCC, bool IND>
+// IND: plans with the multi-bounce term.  FAR: shadow queries visit children
+// far first (same radiance; fewer node visits on some scenes).
+template <bool RECORD, bool COUNT, int OCC, bool IND, bool FAR = false>
 __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
 {
     __shared__ uint32_t lds[kLdsDepth * 256];
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
         if (probe < a.P && dd.w != 0.0f) {
             const float4 pp = a.probes[probe];
             int hp = -1;
-            L = shade_ray<COUNT, IND>(a.sc, a.sp, make_float3(pp.x, pp.y, pp.z),
+            L = shade_ray<COUNT, IND, FAR>(a.sc, a.sp, make_float3(pp.x, pp.y, pp.z),
                                  make_float3(dd.x, dd.y, dd.z), a.tmin, a.tmax, st, cnt, nshadow, hp);
             if (RECORD) {
                 const size_t ri = (size_t)a.probe_perm[probe] * a.R + a.dir_perm[r];
@@ -219,26 +220,38 @@ int occupancy_of(K kernel, int cus)
 
 }  // namespace
 
-int trace_grid_size(Context& ctx, int occ)
+int trace_grid_size(Context& ctx, int occ, bool far)
 {
     int g;
-    if (occ == 8) g = occupancy_of(k_trace<false, false, 8, false>, ctx.num_cus);
+    if (far) {
+        if (occ == 8) g = occupancy_of(k_trace<false, false, 8, false, true>, ctx.num_cus);
+        else if (occ == 6) g = occupancy_of(k_trace<false, false, 6, false, true>, ctx.num_cus);
+        else g = occupancy_of(k_trace<false, false, 0, false, true>, ctx.num_cus);
+    } else if (occ == 8) g = occupancy_of(k_trace<false, false, 8, false>, ctx.num_cus);
     else if (occ == 6) g = occupancy_of(k_trace<false, false, 6, false>, ctx.num_cus);
     else g = occupancy_of(k_trace<false, false, 0, false>, ctx.num_cus);
     return (g + 7) & ~7;
 }
 
-void launch_trace(const BakeArgs& a, bool record, bool count, int occ, int grid, hipStream_t st)
+void launch_trace(const BakeArgs& a, bool record, bool count, int occ, bool far, int grid,
+                  hipStream_t st)
 {
     VLB_REQUIRE(grid % 8 == 0, VLB_ERR_INVALID, "trace grid must be a multiple of 8");
     k_zero_ctr8<<<1, 64, 0, st>>>(a.wctr);
+    // recording (hit ids are the same in any shadow order) and multi-bounce
+    // plans keep slot-order shadows
     if (a.sp.ind.coeffs) {
         if (record) k_trace<true, false, 0, true><<<grid, 256, 0, st>>>(a);
         else if (count) k_trace<false, true, 0, true><<<grid, 256, 0, st>>>(a);
         else k_trace<false, false, 0, true><<<grid, 256, 0, st>>>(a);
     } else if (record) k_trace<true, false, 0, false><<<grid, 256, 0, st>>>(a);
+    else if (count && far) k_trace<false, true, 0, false, true><<<grid, 256, 0, st>>>(a);
     else if (count) k_trace<false, true, 0, false><<<grid, 256, 0, st>>>(a);
-    else if (occ == 8) k_trace<false, false, 8, false><<<grid, 256, 0, st>>>(a);
+    else if (far) {
+        if (occ == 8) k_trace<false, false, 8, false, true><<<grid, 256, 0, st>>>(a);
+        else if (occ == 6) k_trace<false, false, 6, false, true><<<grid, 256, 0, st>>>(a);
+        else k_trace<false, false, 0, false, true><<<grid, 256, 0, st>>>(a);
+    } else if (occ == 8) k_trace<false, false, 8, false><<<grid, 256, 0, st>>>(a);
     else if (occ == 6) k_trace<false, false, 6, false><<<grid, 256, 0, st>>>(a);
     else k_trace<false, false, 0, false><<<grid, 256, 0, st>>>(a);
     VLB_HIP(hipGetLastError());

@@ -187,10 +187,11 @@ void RoundBufs::alloc(Context& ctx, size_t cap, int nv, uint32_t dep)
     nvis = nv;
     depth = dep;
     grid = multi_grid_size(ctx);
-    fgrid[0] = multi_fused_grid_size(ctx, 0);
-    fgrid[1] = multi_fused_grid_size(ctx, 6);
-    fgrid[2] = multi_fused_grid_size(ctx, 8);
-    const int max_grid = std::max(std::max(grid, fgrid[0]), std::max(fgrid[1], fgrid[2]));
+    int max_grid = grid;
+    for (int k = 0; k < 6; ++k) {
+        fgrid[k] = multi_fused_grid_size(ctx, k % 3 == 0 ? 0 : (k % 3 == 1 ? 6 : 8), k >= 3);
+        max_grid = std::max(max_grid, fgrid[k]);
+    }
     shade_grid = (int)std::max<size_t>(8, std::min<size_t>(cap * 2, 8192));
     pgrid = ctx.num_cus * 8;
     mgrid = 1024;
@@ -205,7 +206,7 @@ void RoundBufs::alloc(Context& ctx, size_t cap, int nv, uint32_t dep)
 }
 
 MultiArgs RoundBufs::args(const JobDesc* jobs, const uint8_t* table, const size_t off[3], int map,
-                          int refill, int chunk, int focc) const
+                          int refill, int chunk, int focc, bool far) const
 {
     MultiArgs a;
     memset(&a, 0, sizeof a);
@@ -223,15 +224,18 @@ MultiArgs RoundBufs::args(const JobDesc* jobs, const uint8_t* table, const size_
     a.wctr = wctr.as<unsigned long long>();
     a.spill = spill.as<uint32_t>();
     a.focc = focc;
-    a.fgrid = focc < 0 ? 0 : fgrid[focc == 8 ? 2 : (focc == 6 ? 1 : 0)];
+    a.far = focc >= 0 && far ? 1 : 0;
+    a.fgrid = focc < 0 ? 0 : fgrid[(focc == 8 ? 2 : (focc == 6 ? 1 : 0)) + 3 * a.far];
     a.spill_stride = (uint32_t)(focc < 0 ? grid : a.fgrid) * 256u;
     a.counters = counters.as<unsigned long long>();
     return a;
 }
 
-void multi_variant(uint32_t v, int& map, int& refill, int& focc)
+void multi_variant(uint32_t v, int& map, int& refill, int& focc, bool& far)
 {
-    VLB_REQUIRE(v >= 1 && v <= 24, VLB_ERR_INVALID, "multi-job trace_variant must be 1..24");
+    VLB_REQUIRE(v >= 1 && v <= 36, VLB_ERR_INVALID, "multi-job trace_variant must be 1..36");
+    far = v >= 25;
+    if (far) v -= 24;
     static const int kRefill[3] = {16, 32, 8};
     static const int kOcc[3] = {0, 6, 8};
     if (v >= 13) {
@@ -265,6 +269,7 @@ struct Batch {
     RoundBufs rb;
     uint32_t variant = kBatchDefaultVariant;
     int map = 3, refill = 32, chunk = 256, focc = -1;
+    bool far = false;
     std::vector<Scene*> scenes;    // one per job, each holding a use of its scene
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -306,7 +311,7 @@ struct Batch {
     {
         const size_t off[3] = {round_off[r] + parts[r][0], round_off[r] + parts[r][1],
                                round_off[r] + parts[r][2]};
-        return rb.args(jobtab.as<JobDesc>(), table.as<uint8_t>(), off, map, refill, chunk, focc);
+        return rb.args(jobtab.as<JobDesc>(), table.as<uint8_t>(), off, map, refill, chunk, focc, far);
     }
     // E: kset() events to record around the kernels, or null
     void launch_all(bool count, hipStream_t st, hipEvent_t* E = nullptr)
@@ -408,7 +413,7 @@ void build_batch(Context& ctx, const vlb_batch_job* jobs, uint32_t njobs, const 
     uint64_t max_rays = opt && opt->max_round_rays ? opt->max_round_rays : kDefaultRoundRays;
     max_rays = std::min<uint64_t>(max_rays, 1ull << 30);
     b.variant = opt && opt->trace_variant ? opt->trace_variant : kBatchDefaultVariant;
-    multi_variant(b.variant, b.map, b.refill, b.focc);
+    multi_variant(b.variant, b.map, b.refill, b.focc, b.far);
     b.chunk = opt && opt->chunk ? (int)opt->chunk : 256;
     VLB_REQUIRE(b.chunk >= 64 && b.chunk <= 1024 && 1024 % b.chunk == 0, VLB_ERR_INVALID,
                 "chunk must divide 1024 and be at least 64");
@@ -628,19 +633,20 @@ vlb_status vlb_batch_autotune(vlb_batch* h, uint32_t iters, uint32_t* variant, u
         int best_c = b->chunk;
         // wavefront: every lane map x refill x work chunk {256, 1024}; fused
         // (one chunk size: it fetches 256 rays per workgroup): lane maps 2, 3
-        // at 6 and 8 waves/SIMD
+        // at 6 and 8 waves/SIMD, slot-order and far-first shadows
         std::vector<std::pair<uint32_t, int>> cand;
         for (uint32_t v = 13; v <= 24; ++v)
             for (int ch : {256, 1024}) cand.push_back({v, ch});
-        for (uint32_t v : {7u, 8u, 11u, 12u}) cand.push_back({v, 256});
+        for (uint32_t v : {7u, 8u, 11u, 12u, 31u, 32u, 35u, 36u}) cand.push_back({v, 256});
         try {
             for (const auto& vc : cand) {
                 const uint32_t v = vc.first;
                 const int ch = vc.second;
                 int map, refill, focc;
-                multi_variant(v, map, refill, focc);
+                bool far;
+                multi_variant(v, map, refill, focc, far);
                 const MultiArgs a = b->rb.args(b->jobtab.as<JobDesc>(), tab.as<uint8_t>(), off, map,
-                                               refill, ch, focc);
+                                               refill, ch, focc, far);
                 launch_multi_trace(a, false, b->rb.grid, b->rb.shade_grid, st);  // warm-up
                 VLB_HIP(hipEventRecord(e0, st));
                 for (uint32_t i = 0; i < iters; ++i)
@@ -664,7 +670,7 @@ vlb_status vlb_batch_autotune(vlb_batch* h, uint32_t iters, uint32_t* variant, u
         (void)hipEventDestroy(e1);
         VLB_HIP(hipStreamSynchronize(st));
         b->variant = best_v;
-        multi_variant(best_v, b->map, b->refill, b->focc);
+        multi_variant(best_v, b->map, b->refill, b->focc, b->far);
         b->chunk = best_c;
         b->drop_graph();
         if (variant) *variant = best_v;

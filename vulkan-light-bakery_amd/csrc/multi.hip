@@ -8,7 +8,7 @@
 //   k_rays_jobs<true>   shadow query of every (light, ray) pair
 //   k_shade_jobs        radiance per ray (MFMA operand order)
 //   k_project_jobs      SH projection of every projection tile (MFMA)
-// (k_fused_jobs replaces the first three for trace_variant 1..12) and
+// (k_fused_jobs replaces the first three for trace_variant 1..12, 25..36) and
 // k_merge_jobs merges the FP32 (sum, compensation) partials of finished
 // jobs in FP64.  The counts of a round live in device memory (WorkHeader),
 // so one captured hipGraph replays rounds of any size.
@@ -250,13 +250,14 @@ __global__ __launch_bounds__(256) void k_rays_jobs(MultiArgs a, const JobDesc* _
     }
 }
 
-// Fused form of the three passes above (trace_variant 1..12): each lane takes
-// one primary ray and runs closest hit, shading and its shadow rays with the
-// single-bake kernel's shade_ray, writing radiance in MFMA operand order.  A
+// Fused form of the three passes above (trace_variant 1..12; 25..36 with FAR,
+// far-first shadow queries): each lane takes one primary ray and runs closest
+// hit, shading and its shadow rays with the single-bake kernel's shade_ray,
+// writing radiance in MFMA operand order.  A
 // wave fetches 64 items at a time (a chunk lies within a pair of trace tiles,
 // hence one segment and one job).  Same values as the wavefront passes, bit
 // for bit.
-template <bool COUNT, int OCC>
+template <bool COUNT, int OCC, bool FAR = false>
 __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_fused_jobs(MultiArgs a,
                                                                        const JobDesc* __restrict__ jobs,
                                                                        const Segment* __restrict__ seg,
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_fused_jobs(MultiArgs
         if (probe < J.P && dd.w != 0.0f) {
             const float4 pp = J.probes[probe];
             int hp = -1;
-            L = shade_ray<COUNT>(J.sc, J.sp, make_float3(pp.x, pp.y, pp.z), make_float3(dd.x, dd.y, dd.z),
+            L = shade_ray<COUNT, false, FAR>(J.sc, J.sp, make_float3(pp.x, pp.y, pp.z), make_float3(dd.x, dd.y, dd.z),
                                  J.tmin, J.tmax, st, cnt, nshadow, hp);
         }
         float* out = a.rad + (size_t)tile * 1536 + l * 8 + e;
@@ -436,10 +437,14 @@ int occupancy_of(K kernel, int cus)
 
 }  // namespace
 
-int multi_fused_grid_size(Context& ctx, int occ)
+int multi_fused_grid_size(Context& ctx, int occ, bool far)
 {
     int g;
-    if (occ == 8) g = occupancy_of(k_fused_jobs<false, 8>, ctx.num_cus);
+    if (far) {
+        if (occ == 8) g = occupancy_of(k_fused_jobs<false, 8, true>, ctx.num_cus);
+        else if (occ == 6) g = occupancy_of(k_fused_jobs<false, 6, true>, ctx.num_cus);
+        else g = occupancy_of(k_fused_jobs<false, 0, true>, ctx.num_cus);
+    } else if (occ == 8) g = occupancy_of(k_fused_jobs<false, 8>, ctx.num_cus);
     else if (occ == 6) g = occupancy_of(k_fused_jobs<false, 6>, ctx.num_cus);
     else g = occupancy_of(k_fused_jobs<false, 0>, ctx.num_cus);
     return (g + 7) & ~7;
@@ -460,8 +465,13 @@ void launch_multi_trace(const MultiArgs& a, bool count, int grid, int shade_grid
     k_zero_ctr<<<1, 64, 0, st>>>(a.wctr, 32);
     if (a.focc >= 0) {
         VLB_REQUIRE(a.fgrid % 8 == 0 && a.fgrid > 0, VLB_ERR_INVALID, "fused grid must be a multiple of 8");
-        if (count) k_fused_jobs<true, 0><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
-        else if (a.focc == 8) k_fused_jobs<false, 8><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
+        if (count && a.far) k_fused_jobs<true, 0, true><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
+        else if (count) k_fused_jobs<true, 0><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
+        else if (a.far) {
+            if (a.focc == 8) k_fused_jobs<false, 8, true><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
+            else if (a.focc == 6) k_fused_jobs<false, 6, true><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
+            else k_fused_jobs<false, 0, true><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
+        } else if (a.focc == 8) k_fused_jobs<false, 8><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
         else if (a.focc == 6) k_fused_jobs<false, 6><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
         else k_fused_jobs<false, 0><<<a.fgrid, 256, 0, st>>>(a, a.jobs, a.seg, a.hdr);
         VLB_HIP(hipGetLastError());

@@ -80,20 +80,21 @@ struct RoundBufs {
     int nvis = 0;             // visibility bytes per ray
     uint32_t depth = 0;       // deepest BVH served
     int grid = 0, shade_grid = 0, pgrid = 0, mgrid = 0;
-    int fgrid[3] = {0, 0, 0};   // fused kernel grids at occupancy 0, 6, 8
+    int fgrid[6] = {};          // fused kernel grids at occupancy 0, 6, 8 (+3: far-first shadows)
     DevBuf rad, hits, vis, wctr, spill, counters;
     void alloc(Context& ctx, size_t cap_tiles, int nvis, uint32_t depth);
-    // kernel arguments for the given device tables
     // kernel arguments for the given device tables; focc: -1 for the
-    // wavefront kernels, else the fused kernel's occupancy (0, 6, 8)
+    // wavefront kernels, else the fused kernel's occupancy (0, 6, 8); far:
+    // the fused kernel's shadow queries far first
     MultiArgs args(const JobDesc* jobs, const uint8_t* table, const size_t off[3], int map,
-                   int refill, int chunk, int focc) const;
+                   int refill, int chunk, int focc, bool far) const;
 };
 
-// trace_variant -> lane map, refill threshold and fused occupancy: 1..12 the
-// fused kernel (1 + map + 4 * occupancy index, occupancy {0, 6, 8}; focc =
-// that occupancy), 13..24 the wavefront kernels (13 + map + 4 * refill
-// index, refill {16, 32, 8}; focc = -1).
-void multi_variant(uint32_t v, int& map, int& refill, int& focc);
+// trace_variant -> lane map, refill threshold, fused occupancy and shadow
+// order: 1..12 the fused kernel (1 + map + 4 * occupancy index, occupancy
+// {0, 6, 8}; focc = that occupancy), 13..24 the wavefront kernels (13 + map +
+// 4 * refill index, refill {16, 32, 8}; focc = -1), 25..36 as 1..12 with
+// far-first shadow queries (far = true).
+void multi_variant(uint32_t v, int& map, int& refill, int& focc, bool& far);
 
 }  // namespace vlb

@@ -164,6 +164,7 @@ struct Server {
     RayCache cache;
     RoundBufs rb;
     int map = 3, refill = 32, chunk = 256, focc = -1;
+    bool far = false;
     size_t cap_tiles = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -387,7 +388,7 @@ void Server::launch_step(const RoundTable& rt, const std::vector<Upload>& ups, u
         VLB_HIP(hipMemcpyAsync(u.dst, stg + u.src, u.bytes, hipMemcpyHostToDevice, st));
     if (!gexec) {
         const MultiArgs a = rb.args(dtab.as<JobDesc>(), dtab.as<uint8_t>() + jobtab_bytes, off, map, refill, chunk,
-                                    focc);
+                                    focc, far);
         VLB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         try {
             launch_multi_trace(a, false, rb.grid, rb.shade_grid, st);
@@ -633,7 +634,7 @@ vlb_status vlb_server_create(vlb_ctx* h, const vlb_server_options* o, vlb_server
         VLB_REQUIRE(op.policy <= VLB_SCHED_FIFO, VLB_ERR_INVALID, "bad scheduling policy");
         VLB_REQUIRE(op.age_ms == 0 || op.policy == VLB_SCHED_SRPT, VLB_ERR_INVALID,
                     "age_ms applies to the SRPT policy only");
-        multi_variant(op.trace_variant, s->map, s->refill, s->focc);
+        multi_variant(op.trace_variant, s->map, s->refill, s->focc, s->far);
         s->chunk = (int)op.chunk;
         VLB_REQUIRE(s->chunk >= 64 && s->chunk <= 1024 && 1024 % s->chunk == 0, VLB_ERR_INVALID,
                     "chunk must divide 1024 and be at least 64");

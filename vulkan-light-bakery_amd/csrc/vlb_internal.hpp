@@ -291,6 +291,7 @@ struct Plan {
     int occ = 0;              // k_trace waves-per-SIMD variant (0, 6, 8)
     uint32_t variant = 0;     // trace_variant in use
     bool wave = false;        // wavefront pipeline (variants 13..24)
+    bool far = false;         // far-first shadow queries (variants 25..36)
     int wgrid = 0;            // k_rays grid
     DevBuf hits, vis, wctr;
     int pgrid = 0;            // projection grid
@@ -382,6 +383,7 @@ struct MultiArgs {
     int focc;                 // -1: wavefront kernels; else the fused kernel at this
                               // occupancy (0 = compiler's choice, 6, 8 waves/SIMD)
     int fgrid;                // grid of the fused kernel
+    int far;                  // fused kernel: far-first shadow queries
 };
 
 // ------------------------------------------------------ probe-grid lookup
@@ -407,8 +409,9 @@ void build_scene_bvh(Context& ctx, Scene& s, const float* xyz, uint32_t nverts,
                      const uint32_t* idx, uint32_t ntris, const float* normals,
                      const float* albedo);
 // bake.hip
-int  trace_grid_size(Context& ctx, int occ);
-void launch_trace(const BakeArgs& a, bool record, bool count, int occ, int grid, hipStream_t stream);
+int  trace_grid_size(Context& ctx, int occ, bool far);
+void launch_trace(const BakeArgs& a, bool record, bool count, int occ, bool far, int grid,
+                  hipStream_t stream);
 void launch_project(const BakeArgs& a, int precision, int grid, hipStream_t stream);
 void launch_copy_f32(float* dst, const float* src, size_t n, hipStream_t stream);
 // wave.hip
@@ -423,7 +426,7 @@ void launch_trace_occluded(const SceneView& sc, const float* org, const float* d
                            uint32_t* spill, uint32_t spill_stride, int grid, hipStream_t st);
 // multi.hip
 int  multi_grid_size(Context& ctx);
-int  multi_fused_grid_size(Context& ctx, int occ);
+int  multi_fused_grid_size(Context& ctx, int occ, bool far);
 void launch_multi_trace(const MultiArgs& a, bool count, int grid, int shade_grid, hipStream_t st);
 void launch_multi_project(const MultiArgs& a, int precision, int grid, hipStream_t st);
 void launch_multi_merge(const MultiArgs& a, int grid, hipStream_t st);

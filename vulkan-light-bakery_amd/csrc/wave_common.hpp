@@ -169,7 +169,7 @@ __device__ __forceinline__ void visit_node(TravState& T, const QNode4* nodes, co
         q.lds[T.nq * 256] = (uint32_t)~c[k];
         T.nq += (int)((leaves >> k) & 1u);
     }
-    T.live = next_node<!ANY>(e, c, st, T.sp, T.node);  // shadow queries: slot order
+    T.live = next_node<ANY ? kOrderSlot : kOrderNear>(e, c, st, T.sp, T.node);
 }
 
 // Tests every queued triangle of the active lanes (the wave runs as many
