@@ -149,6 +149,13 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v)
     return v;
 }
 
+// 30-bit Morton code of each triangle's box centre on a grid of cubic cells:
+// 1024 cells along the longest extent of the centre bounds, the same cell
+// size on the other axes.  With cells stretched to 1024 per axis (2.5-6x
+// finer vertically on the flat terrains and rooms) the PLOC pass built
+// trees that traced 2-6% slower on three of the four scenes of the batched
+// jobs and the same on the fourth (208.3 against 202.4 ms per batch); the
+// c2 room traces 2% faster with them (0.99 against 1.01 ms).
 __global__ void k_morton(const float4* __restrict__ cen, int n, const float* __restrict__ bounds,
                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
@@ -157,8 +164,8 @@ __global__ void k_morton(const float4* __restrict__ cen, int n, const float* __r
     float4 c = cen[i];
     float q[3];
     const float cc[3] = {c.x, c.y, c.z};
+    const float ext = fmaxf(fmaxf(bounds[3] - bounds[0], bounds[4] - bounds[1]), bounds[5] - bounds[2]);
     for (int a = 0; a < 3; ++a) {
-        float ext = bounds[3 + a] - bounds[a];
         float x = ext > 0.0f ? (cc[a] - bounds[a]) / ext * 1024.0f : 0.0f;
         q[a] = fminf(fmaxf(x, 0.0f), 1023.0f);
     }
