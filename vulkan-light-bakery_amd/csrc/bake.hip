@@ -86,8 +86,14 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
                 ++victim;
             }
         }
-        b = ((unsigned long long)__shfl((unsigned)(b >> 32), 0) << 32) | (unsigned long long)__shfl((unsigned)b, 0);
-        victim = __shfl(victim, 0);
+        // every lane is active here, so the first active lane is lane 0: the
+        // fetch position and range count become scalars.  Broadcast with
+        // __shfl they stayed per-lane values: 34 instead of 26 spilled VGPRs
+        // at 8 waves/SIMD and a c2 trace stage 0.5-2% longer.  (The
+        // multi-job kernel measured the same either way and keeps __shfl.)
+        b = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(b >> 32)) << 32) |
+            (unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)b);
+        victim = __builtin_amdgcn_readfirstlane(victim);
         if (b == ~0ull) break;
         const size_t it = (size_t)b + lane;
         int tile, l, e;
