@@ -346,7 +346,10 @@ def bench_c4(args):
         variant, chunk = args.variant, 256
     else:
         b = ctx.batch(jobs16)
-        variant, chunk = b.autotune(2)
+        # 8 timed runs per candidate on the library's ~16M-ray sample (about
+        # 2 ms each): variants 12 and 36 are 1% apart on these jobs, and 2
+        # runs let run-to-run noise pick either
+        variant, chunk = b.autotune(8)
     b.count()
     cnt = b.info()
     # timed region 1 (`value`): the batch as one captured hipGraph per step

@@ -58,6 +58,7 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
     st.lds = &lds[tid];
     st.spill = a.spill + (size_t)blockIdx.x * 256 + tid;
     st.spill_stride = a.spill_stride;
+    st.depth = kLdsDepth;
     TraceCount cnt;
     uint32_t nshadow = 0;
     // 32 64-aligned ranges, range 4x + w owned by wave w of the workgroups on
@@ -184,6 +185,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(SceneView sc, const float
     st.lds = &lds[threadIdx.x];
     st.spill = spill + (size_t)blockIdx.x * 256 + threadIdx.x;
     st.spill_stride = stride;
+    st.depth = kLdsDepth;
     TraceCount cnt;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const float3 o = make_float3(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
@@ -208,6 +210,7 @@ __global__ __launch_bounds__(256) void k_trace_occluded(SceneView sc, const floa
     st.lds = &lds[threadIdx.x];
     st.spill = spill + (size_t)blockIdx.x * 256 + threadIdx.x;
     st.spill_stride = stride;
+    st.depth = kLdsDepth;
     TraceCount cnt;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const float3 o = make_float3(org[3 * i], org[3 * i + 1], org[3 * i + 2]);

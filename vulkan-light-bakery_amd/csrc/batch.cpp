@@ -201,7 +201,7 @@ void RoundBufs::alloc(Context& ctx, size_t cap, int nv, uint32_t dep)
     wctr = dev_alloc(ctx, 32 * kCtrStride * sizeof(unsigned long long));
     counters = dev_alloc(ctx, 4 * sizeof(unsigned long long));
     const uint32_t sb = stack_bound(dep);
-    const uint32_t e = sb > (uint32_t)kLdsDepth ? sb - kLdsDepth : 0;
+    const uint32_t e = sb > (uint32_t)kLdsDepthMin ? sb - kLdsDepthMin : 0;
     spill = dev_alloc(ctx, std::max<size_t>(16, (size_t)e * max_grid * 256 * 4));
 }
 

@@ -95,7 +95,14 @@ struct ShadeParams {
     ProbeGridView ind;
 };
 
-constexpr int kLdsDepth = 16;  // stack entries kept in LDS per lane
+// Stack entries kept in LDS per lane: 16 for the fused and single-query
+// kernels, 12 for the wavefront kernels, whose leaf queues take another 8 KB
+// per workgroup (12 + 8 entries x 256 lanes x 4 B = 20 KB lets them run 7
+// waves/SIMD: c3 trace 2.5% faster; the fused kernels measured 1% slower at
+// 12 on c2).  Spill areas are sized for the smaller depth.
+constexpr int kLdsDepth = 16;
+constexpr int kLdsDepthWave = 12;
+constexpr int kLdsDepthMin = kLdsDepthWave < kLdsDepth ? kLdsDepthWave : kLdsDepth;
 // Work counters (persistent kernels' fetch positions) sit 128 bytes apart,
 // one per cache line: device-scope atomics on one line serialise.
 constexpr int kCtrStride = 16;
@@ -248,26 +255,27 @@ __device__ __forceinline__ float slab(const SlabRay& s, float lox, float loy, fl
 }
 
 // ------------------------------------------------------------- traversal
-// Stack: entries [0, kLdsDepth) in LDS at lds[i * 256] (lane-interleaved,
+// Stack: entries [0, depth) in LDS at lds[i * 256] (lane-interleaved,
 // conflict-free), deeper entries in a global spill area at
-// spill[(i - kLdsDepth) * spill_stride].
+// spill[(i - depth) * spill_stride].
 struct Stack {
     uint32_t* lds;
     uint32_t* spill;
     uint32_t  spill_stride;
+    int       depth;  // LDS entries per lane (a constant of the kernel)
 };
 
 __device__ __forceinline__ void push(const Stack& st, int& sp, uint32_t v)
 {
-    if (sp < kLdsDepth) st.lds[sp * 256] = v;
-    else st.spill[(size_t)(sp - kLdsDepth) * st.spill_stride] = v;
+    if (sp < st.depth) st.lds[sp * 256] = v;
+    else st.spill[(size_t)(sp - st.depth) * st.spill_stride] = v;
     ++sp;
 }
 __device__ __forceinline__ uint32_t pop(const Stack& st, int& sp)
 {
     --sp;
-    if (sp < kLdsDepth) return st.lds[sp * 256];
-    return st.spill[(size_t)(sp - kLdsDepth) * st.spill_stride];
+    if (sp < st.depth) return st.lds[sp * 256];
+    return st.spill[(size_t)(sp - st.depth) * st.spill_stride];
 }
 
 struct TraceCount {
@@ -496,7 +504,7 @@ __device__ __forceinline__ bool next_node(float e[4], int c[4], const Stack& st,
         cswap(e[1], c[1], e[2], c[2]);
     }
     const float inf = __builtin_huge_valf();
-    if (__all(sp + 3 <= kLdsDepth)) {
+    if (__all(sp + 3 <= st.depth)) {
         // Every active lane has three free LDS entries (one wave-uniform
         // test): straight-line pushes with conditional increments.  A write
         // past the top lands at most where the all-hit case writes, and is

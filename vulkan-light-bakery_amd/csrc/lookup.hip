@@ -22,6 +22,7 @@ __global__ __launch_bounds__(256) void k_lookup(LookupArgs a)
     st.lds = &lds[threadIdx.x];
     st.spill = a.spill + (size_t)blockIdx.x * 256 + threadIdx.x;
     st.spill_stride = a.spill_stride;
+    st.depth = kLdsDepth;
     TraceCount cnt;
     const ProbeGridView& g = a.g;
     const size_t total = (size_t)a.n * 8;

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_rays_jobs(MultiArgs a, const JobDesc* _
                                                    const Segment* __restrict__ seg,
                                                    const WorkHeader* __restrict__ hdr)
 {
-    __shared__ uint32_t lds[kLdsDepth * 256];
+    __shared__ uint32_t lds[kLdsDepthWave * 256];
     __shared__ uint32_t qlds[kLeafQueue * 256];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -154,6 +154,7 @@ __global__ __launch_bounds__(256) void k_rays_jobs(MultiArgs a, const JobDesc* _
     q.lds = &qlds[tid];
     st.spill = a.spill + (size_t)blockIdx.x * 256 + tid;
     st.spill_stride = a.spill_stride;
+    st.depth = kLdsDepthWave;
     TraceCount cnt;
     uint32_t ntraced = 0;
 
@@ -276,6 +277,7 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_fused_jobs(MultiArgs
     st.lds = &lds[tid];
     st.spill = a.spill + (size_t)blockIdx.x * 256 + tid;
     st.spill_stride = a.spill_stride;
+    st.depth = kLdsDepth;
     TraceCount cnt;
     uint32_t nshadow = 0;
     const int nseg = hdr->nseg;

@@ -159,7 +159,7 @@ void validate(const vlb_bake_params& p)
 size_t scene_spill_bytes(const Scene& s, uint32_t stride)
 {
     const uint32_t sb = stack_bound(s.depth);
-    const uint32_t e = sb > (uint32_t)kLdsDepth ? sb - kLdsDepth : 0;
+    const uint32_t e = sb > (uint32_t)kLdsDepthMin ? sb - kLdsDepthMin : 0;
     return std::max<size_t>(16, (size_t)e * stride * 4);
 }
 

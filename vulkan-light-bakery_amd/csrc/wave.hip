@@ -126,7 +126,7 @@ __device__ __forceinline__ bool item_setup(const BakeArgs& a, size_t it, TravSta
 template <bool ANY, bool COUNT, bool CORNERS>
 __global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigned long long* ctr)
 {
-    __shared__ uint32_t lds[kLdsDepth * 256];
+    __shared__ uint32_t lds[kLdsDepthWave * 256];
     __shared__ uint32_t qlds[kLeafQueue * 256];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -136,6 +136,7 @@ __global__ __launch_bounds__(256) void k_rays(BakeArgs a, size_t nitems, unsigne
     q.lds = &qlds[tid];
     st.spill = a.spill + (size_t)blockIdx.x * 256 + tid;
     st.spill_stride = a.spill_stride;
+    st.depth = kLdsDepthWave;
     TraceCount cnt;
     uint32_t ntraced = 0;
     // eight kChunk-aligned ranges of the item space, one per XCD; a wave
