@@ -262,7 +262,7 @@ struct Stack {
     uint32_t* lds;
     uint32_t* spill;
     uint32_t  spill_stride;
-    int       depth;  // LDS entries per lane (a constant of the kernel)
+    int       depth = kLdsDepth;  // LDS entries per lane (a constant of the kernel)
 };
 
 __device__ __forceinline__ void push(const Stack& st, int& sp, uint32_t v)
