@@ -122,6 +122,37 @@ def test_bvh_coincident_and_degenerate_triangles(ctx):
     ctx.release_scene(sid)
 
 
+def test_deep_tree_bakes_identical_across_kernel_families(ctx):
+    """A deep BVH (thousands of coincident and degenerate triangles among
+    random ones) makes traversal stacks run past the LDS entries of every
+    kernel family (16 in the fused kernels, 12 in the wavefront ones) into
+    their global spill areas: every family bakes the same bits, and the
+    oracle's coefficients."""
+    rng = np.random.default_rng(19)
+    one = np.array([[0, 0, 0], [4, 0, 0], [0, 4, 0]], np.float32)
+    point = np.full((3, 3), 2.0, np.float32)
+    rand = rng.uniform(-6, 6, size=(400, 3, 3)).astype(np.float32)
+    tris = np.concatenate([np.repeat(one[None], 3000, 0), np.repeat(point[None], 3000, 0), rand])
+    tris = tris[rng.permutation(len(tris))]
+    xyz = tris.reshape(-1, 3)
+    idx = np.arange(len(xyz), dtype=np.uint32).reshape(-1, 3)
+    sid = ctx.upload_scene(xyz, idx)
+    # stack bound 3 * ceil(depth / 2) + 1 >= 19 entries: past both LDS depths
+    assert ctx.scene_info(sid)["depth"] >= 12
+    o = orc.Scene(xyz, idx)
+    probes = rng.uniform([-1, -1, -2], [5, 5, 2], size=(24, 3)).astype(np.float32)
+    p = _bake_params(probes, rays=400)
+    ref = None
+    for v in (12, 36, 16, 20, 1):
+        p.trace_variant = v
+        g = ctx.bake(sid, p)
+        if ref is None:
+            ref = g
+            assert rel_err(g, o.bake(p, mode64=True)["coeffs"]).max() <= 1e-5
+        assert np.array_equal(g.view(np.uint32), ref.view(np.uint32)), v
+    ctx.release_scene(sid)
+
+
 def _bake_params(probes, rays=1024, **kw):
     rng = np.random.default_rng(7)
     sun = vlb.sun_direction(rng)
