@@ -155,7 +155,8 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v)
 // finer vertically on the flat terrains and rooms) the PLOC pass built
 // trees that traced 2-6% slower on three of the four scenes of the batched
 // jobs and the same on the fourth (208.3 against 202.4 ms per batch); the
-// c2 room traces 2% faster with them (0.99 against 1.01 ms).
+// 11k-triangle room bake of 2048 probes traces 2% faster with them (0.99
+// against 1.01 ms).
 __global__ void k_morton(const float4* __restrict__ cen, int n, const float* __restrict__ bounds,
                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
@@ -190,7 +191,8 @@ __global__ void k_morton(const float4* __restrict__ cen, int n, const float* __r
 // final merge -- the root -- is node 0.  Search radius 32: on the batched
 // jobs (rooms and terrains of 11k-1.15M triangles) 1.5% fewer node visits
 // than radius 16 and a 1.3% shorter trace stage; 24 and 48 were no better,
-// and the c2 room and the c3 terrain trace in the same time at 16-64.
+// and the 11k-triangle room and the 1.15M-triangle terrain bakes trace in
+// the same time at 16-64.
 constexpr int kPlocRadius = 32;
 
 // Leaf-order triangles and the initial clusters (leaf j = ~j).

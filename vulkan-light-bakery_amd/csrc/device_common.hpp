@@ -98,8 +98,10 @@ struct ShadeParams {
 // Stack entries kept in LDS per lane: 16 for the fused and single-query
 // kernels, 12 for the wavefront kernels, whose leaf queues take another 8 KB
 // per workgroup (12 + 8 entries x 256 lanes x 4 B = 20 KB lets them run 7
-// waves/SIMD: c3 trace 2.5% faster; the fused kernels measured 1% slower at
-// 12 on c2).  Spill areas are sized for the smaller depth.
+// waves/SIMD: the trace stage of a 1.15M-triangle terrain bake, 16,384
+// probes x 16,384 rays, 2.5% faster; the fused kernels measured 1% slower at
+// 12 on an 11k-triangle room bake).  Spill areas are sized for the smaller
+// depth.
 constexpr int kLdsDepth = 16;
 constexpr int kLdsDepthWave = 12;
 constexpr int kLdsDepthMin = kLdsDepthWave < kLdsDepth ? kLdsDepthWave : kLdsDepth;
@@ -342,7 +344,8 @@ __device__ __forceinline__ BNode4 load_node(const QNode4* nodes, uint32_t i)
 // the divergent fetches keep busy, to those.  Nodes are read-only while any
 // trace kernel runs.  Used by the fused traversal (trace_closest, trace_any);
 // the wavefront kernels refill lanes with unrelated rays, rarely share a
-// node across a wave and measured slower with it (c2 1.48 vs 1.41 ms).
+// node across a wave and measured slower with it (1.48 vs 1.41 ms on the
+// 11k-triangle room bake, 2048 probes x 4096 rays).
 typedef int sgpr16_t __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ void scalar_load64(unsigned long long addr, sgpr16_t& a)
@@ -482,11 +485,15 @@ enum : int {
 // pop.  Returns false when the stack is empty.  Closest-hit queries sort near
 // first.  Shadow queries stop at any hit, so the order only moves the visit
 // count: slot order drops the sort's 20 instructions per visit and visits
-// about as many nodes as near first (0.1-0.3% fewer without the sort on the
-// c2-c4 scenes); far first (the sort on negated entries, misses staying +inf
-// and last) reaches an occluder sooner on some scenes: 5.1% fewer visits on
-// the c2 scene, which pays for the sort (4% faster), 1.4% fewer on the c4
-// mix, which does not (0.5% slower).  The pushes below are right for any
+// about as many nodes as near first (0.1-0.3% fewer without the sort on
+// rooms and terrains of 11k-1.15M triangles); far first (the sort on negated
+// entries, misses staying +inf and last) reaches an occluder sooner on some
+// trees and not on others, so it is a variant for the autotuners: on the
+// 11k-triangle room it visited 5.1% fewer nodes and ran 4% faster, on a
+// batch of 64 jobs over four scenes 1.4% fewer and 0.5% slower, both with
+// trees built from per-axis Morton cells; on trees from cubic cells (the
+// build's) it runs level with slot order or 1% behind.  The pushes below
+// are right for any
 // order (a missed child is written and then overwritten).
 template <int ORDER = kOrderNear>
 __device__ __forceinline__ bool next_node(float e[4], int c[4], const Stack& st, int& sp,
