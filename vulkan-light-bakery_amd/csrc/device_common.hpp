@@ -493,8 +493,7 @@ enum : int {
 // batch of 64 jobs over four scenes 1.4% fewer and 0.5% slower, both with
 // trees built from per-axis Morton cells; on trees from cubic cells (the
 // build's) it runs level with slot order or 1% behind.  The pushes below
-// are right for any
-// order (a missed child is written and then overwritten).
+// are right for any order (a missed child is written and then overwritten).
 template <int ORDER = kOrderNear>
 __device__ __forceinline__ bool next_node(float e[4], int c[4], const Stack& st, int& sp,
                                           uint32_t& node)
