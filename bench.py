@@ -316,7 +316,7 @@ def serve(ctx, vlb, W, host, sids, args, oracle_of=None):
                       "span_s": round(srv_span, 3), "policy": args.policy,
                       "state_budget_mib": args.state_budget, "step_rays": args.step_rays,
                       "age_ms": args.age_ms,
-                      "trace_variant": args.variant or 12},
+                      "trace_variant": st["trace_variant"]},
            "plain_fp32_per_request": {"probes_per_s": round(P_all / plain_span, 1),
                                       "p50_ms": round(float(np.percentile(plat, 50)), 2),
                                       "p99_ms": round(float(np.percentile(plat, 99)), 2),
@@ -512,7 +512,8 @@ def bench_c5(args):
             q = vlb.Params(**{**ps32[i].__dict__, "_keep": []})
             q.probes = pr[np.sort(rng.choice(pr.shape[0], size=2, replace=False))].copy()
             picks.append((si, q))
-        hits = _hit_parity(ctx, vlb, host, sids, oracle_of, picks, args.variant or 12)
+        # the variant the server ran (its default is 11)
+        hits = _hit_parity(ctx, vlb, host, sids, oracle_of, picks, res["server"]["trace_variant"])
         spp_scene, t_cpu = {}, 0.0
         for i in range(n):
             si = reqs[i][0]

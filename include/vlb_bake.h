@@ -412,7 +412,7 @@ typedef struct vlb_server_options {
     uint32_t max_active;      /* requests on the device at once; 0 = 64   */
     uint32_t max_lights;      /* shadow rays per primary ray; 0 = 4       */
     int32_t  precision;       /* VLB_PREC_* of every request              */
-    uint32_t trace_variant;   /* 1..36 as in vlb_batch_options; 0 = 12    */
+    uint32_t trace_variant;   /* 1..36 as in vlb_batch_options; 0 = 11    */
     uint32_t chunk;           /* rays per work fetch; 0 = 256             */
     uint32_t max_depth;       /* deepest scene BVH served; 0 = 64         */
     uint32_t policy;          /* VLB_SCHED_*: how a step's rays are shared */
@@ -442,7 +442,7 @@ typedef struct vlb_server_stats {
     uint32_t active_peak;     /* most requests admitted at once           */
     uint32_t queued;          /* waiting for admission now                */
     int32_t  status;          /* VLB_OK unless the scheduler failed       */
-    uint32_t reserved;
+    uint32_t trace_variant;   /* the trace variant the server runs        */
 } vlb_server_stats;
 vlb_status  vlb_server_create(vlb_ctx* ctx, const vlb_server_options* opt /* may be NULL */,
                               vlb_server** out);

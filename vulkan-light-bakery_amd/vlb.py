@@ -129,7 +129,7 @@ class ServerStats(C.Structure):
                 ("primary_rays", C.c_uint64), ("probes", C.c_uint64),
                 ("state_bytes_peak", C.c_uint64), ("busy_ms", C.c_double),
                 ("clock_ms", C.c_double), ("active_peak", C.c_uint32), ("queued", C.c_uint32),
-                ("status", C.c_int32), ("reserved", C.c_uint32)]
+                ("status", C.c_int32), ("trace_variant", C.c_uint32)]
 
 
 class ProbeGridDesc(C.Structure):
