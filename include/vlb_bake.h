@@ -412,7 +412,7 @@ typedef struct vlb_server_options {
     uint32_t max_active;      /* requests on the device at once; 0 = 64   */
     uint32_t max_lights;      /* shadow rays per primary ray; 0 = 4       */
     int32_t  precision;       /* VLB_PREC_* of every request              */
-    uint32_t trace_variant;   /* 1..36 as in vlb_batch_options; 0 = 11    */
+    uint32_t trace_variant;   /* 1..36 as in vlb_batch_options; 0 = 35    */
     uint32_t chunk;           /* rays per work fetch; 0 = 256             */
     uint32_t max_depth;       /* deepest scene BVH served; 0 = 64         */
     uint32_t policy;          /* VLB_SCHED_*: how a step's rays are shared */

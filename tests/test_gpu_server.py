@@ -65,7 +65,8 @@ def _bits(a):
     return np.ascontiguousarray(a).view(np.uint32)
 
 
-# 0: the server's default (the fused kernel, variant 11: lane map 2); 12 and 3: the
+# 0: the server's default (the fused kernel, variant 35: lane map 2, far-first
+# shadow queries); 12 and 3: the
 # fused kernel at 8 waves/SIMD and at the compiler's occupancy; 20: the
 # wavefront multi-job kernels (k_rays_jobs / k_shade_jobs, lane map 3); 36:
 # variant 12 with far-first shadow queries
@@ -73,7 +74,7 @@ def _bits(a):
 def test_server_request_equals_single_bake(ctx, scenes, variant):
     reqs = _requests(scenes, n=3)
     srv = vlb.Server(ctx, step_rays=1 << 16, trace_variant=variant)
-    assert srv.stats()["trace_variant"] == (variant or 11)
+    assert srv.stats()["trace_variant"] == (variant or 35)
     for sid, p in reqs:
         got, info = srv.wait(srv.submit(sid, p))
         assert info["status"] == 0 and info["done_ms"] >= info["start_ms"] >= info["admit_ms"]

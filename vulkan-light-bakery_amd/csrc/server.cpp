@@ -621,12 +621,14 @@ vlb_status vlb_server_create(vlb_ctx* h, const vlb_server_options* o, vlb_server
         if (!op.step_rays) op.step_rays = 1ull << 25;
         if (!op.max_active) op.max_active = 64;
         if (!op.max_lights) op.max_lights = 4;
-        // fused, lane map 2 (2 probes x 32 directions per wave), 8 waves/SIMD:
-        // on the request mix of 256-65,536 probes x 4096 rays over rooms,
-        // terrains and a city, 1.6% less device time than lane map 3
-        // (variant 12: 2,054 against 2,088 ms busy for 200 requests) and a
-        // p99 latency of 132-137 against 143-144 ms
-        if (!op.trace_variant) op.trace_variant = 11;
+        // fused, lane map 2 (2 probes x 32 directions per wave), 8 waves/SIMD,
+        // far-first shadow queries: on the request mix of 256-65,536 probes x
+        // 4096 rays over rooms, terrains and a city with moving point lights,
+        // 2.2% less device time than lane map 3 with slot-order shadows
+        // (variant 12: 2,039-2,042 against 2,087-2,089 ms busy for 200
+        // requests; variant 11, slot order, 2,048-2,054) and a p99 latency of
+        // 132-133 against 143-144 ms
+        if (!op.trace_variant) op.trace_variant = 35;
         if (!op.chunk) op.chunk = 256;
         if (!op.max_depth) op.max_depth = 64;
         if (!op.result_bytes) op.result_bytes = 256ull << 20;
