@@ -391,6 +391,55 @@ __device__ __forceinline__ BNode4 load_node_uniform(const QNode4* p)
     return n;
 }
 
+// Watertight test of triangle `leaf` for the fused traversal.  SCALAR: when
+// every active lane tests the same triangle, its record comes through the
+// scalar cache (32 + 16 bytes, exactly the record) and the test runs on the
+// scalar operands, sparing the vector memory path three 16-byte loads per
+// lane; else, and without SCALAR, each lane loads its own.  Each path runs
+// its own test, so the scalar words are never copied into vector registers.
+// The multi-job kernel takes SCALAR (a batch of 64 jobs over four scenes:
+// 197.0 vs 202.9 ms per trace stage); the single-bake kernel does not (on the
+// 11k-triangle room bake it ran 1.03-1.04 vs 0.99-1.00 ms: the extra scalar
+// registers spill there, into vector-register lanes read back in the loop).
+typedef int sgpr8_t __attribute__((ext_vector_type(8)));
+typedef int sgpr4_t __attribute__((ext_vector_type(4)));
+
+template <bool SCALAR>
+__device__ __forceinline__ bool fetch_wt(const float4* tris, int leaf, const WtRay& w, float tmin,
+                                         float tmax, WtHit& h, int& prim)
+{
+    if (!SCALAR) {
+        float3 v0, v1, v2;
+        load_tri(tris, leaf, v0, v1, v2, prim);
+        return wt_test(w, v0, v1, v2, tmin, tmax, h);
+    }
+    const float4* p = tris + 3 * leaf;
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned long long u =
+        ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
+        (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+    if (__all(a == u)) {
+        sgpr8_t x;
+        sgpr4_t y;
+        // early-clobber: the first load's destination never overlaps the
+        // address the second one reads
+        __asm__ __volatile__("s_load_dwordx8 %0, %2, 0x0\n\t"
+                             "s_load_dwordx4 %1, %2, 0x20\n\t"
+                             "s_waitcnt lgkmcnt(0)"
+                             : "=&s"(x), "=&s"(y)
+                             : "s"(u)
+                             : "memory");
+        prim = x[3];
+        return wt_test(w, make_float3(__int_as_float(x[0]), __int_as_float(x[1]), __int_as_float(x[2])),
+                       make_float3(__int_as_float(x[4]), __int_as_float(x[5]), __int_as_float(x[6])),
+                       make_float3(__int_as_float(y[0]), __int_as_float(y[1]), __int_as_float(y[2])),
+                       tmin, tmax, h);
+    }
+    float3 v0, v1, v2;
+    load_tri(tris, leaf, v0, v1, v2, prim);
+    return wt_test(w, v0, v1, v2, tmin, tmax, h);
+}
+
 // Child c[k] of a 4-way node with a runtime k, through selects (a register
 // array indexed at run time would be placed in scratch memory).
 __device__ __forceinline__ int pick4(const int c[4], int k)
@@ -547,7 +596,7 @@ __device__ __forceinline__ bool next_node(float e[4], int c[4], const Stack& st,
 // triangles in [tmin, tmax].  Returns the prim id or -1.  Any BVH whose
 // boxes are conservative gives the same answer: every triangle that could
 // beat the current best is tested, and ties go to the lower prim id.
-template <bool COUNT>
+template <bool COUNT, bool STRI = false>
 __device__ int trace_closest(const SceneView& sc, const float3& o, const float3& d, float tmin,
                              float tmax, float& t_out, float& u_out, float& v_out,
                              const Stack& st, TraceCount& cnt)
@@ -567,13 +616,12 @@ __device__ int trace_closest(const SceneView& sc, const float3& o, const float3&
         // leaves one at a time per lane: the wave runs as many tests as its
         // busiest lane has hit leaves, not one per child slot
         for (uint32_t leaves = hit_leaves; leaves != 0; leaves &= leaves - 1) {
-            float3 v0, v1, v2;
             int prim;
-            load_tri(sc.tris, ~pick4(c, __builtin_ctz(leaves)), v0, v1, v2, prim);
             if (COUNT) ++cnt.tris;
             WtHit h;
             // t <= tbest holds on acceptance; an equal t keeps the lower prim id
-            if (wt_test(w, v0, v1, v2, tmin, tbest, h) && (best < 0 || h.t < tbest || prim < best)) {
+            if (fetch_wt<STRI>(sc.tris, ~pick4(c, __builtin_ctz(leaves)), w, tmin, tbest, h, prim) &&
+                (best < 0 || h.t < tbest || prim < best)) {
                 tbest = h.t;
                 best = prim;
                 bV = h.V;
@@ -593,7 +641,8 @@ __device__ int trace_closest(const SceneView& sc, const float3& o, const float3&
 
 // Shadow query: true if any triangle is accepted in [tmin, tmax].  FAR: the
 // inner children far first, else in slot order (same answer either way).
-template <bool COUNT, bool FAR = false>
+// STRI: uniform triangle records through the scalar cache (fetch_wt).
+template <bool COUNT, bool FAR = false, bool STRI = false>
 __device__ bool trace_any(const SceneView& sc, const float3& o, const float3& d, float tmin,
                           float tmax, const Stack& st, TraceCount& cnt)
 {
@@ -608,12 +657,10 @@ __device__ bool trace_any(const SceneView& sc, const float3& o, const float3& d,
         int c[4];
         const uint32_t hit_leaves = fetch_slab4(sc.nodes, node, s, tmin, tmax, e, c);
         for (uint32_t leaves = hit_leaves; leaves != 0; leaves &= leaves - 1) {
-            float3 v0, v1, v2;
             int prim;
-            load_tri(sc.tris, ~pick4(c, __builtin_ctz(leaves)), v0, v1, v2, prim);
             if (COUNT) ++cnt.tris;
             WtHit h;
-            if (wt_test(w, v0, v1, v2, tmin, tmax, h)) return true;
+            if (fetch_wt<STRI>(sc.tris, ~pick4(c, __builtin_ctz(leaves)), w, tmin, tmax, h, prim)) return true;
         }
         if (!next_node<FAR ? kOrderFar : kOrderSlot>(e, c, st, sp, node)) break;
     }
@@ -650,7 +697,7 @@ __device__ __forceinline__ float unorm8(float x)
 // Indirect term of a multi-bounce bake at hit point P with the two-sided
 // shading normal N: the grid lookup of probe_grid.hpp (visibility rays to the
 // corner probes traced here, counted as shadow rays), times g.scale.
-template <bool COUNT, bool FAR = false>
+template <bool COUNT, bool FAR = false, bool STRI = false>
 __device__ float3 grid_indirect(const SceneView& sc, const ProbeGridView& g, const float3& P,
                                 const float3& N, const Stack& st, TraceCount& cnt,
                                 uint32_t& nshadow)
@@ -667,7 +714,7 @@ __device__ float3 grid_indirect(const SceneView& sc, const ProbeGridView& g, con
             ++nshadow;
             const float3 org = make_float3(P.x + g.bias * N.x, P.y + g.bias * N.y, P.z + g.bias * N.z);
             const float3 dir = make_float3(c.dir[0] / c.dist, c.dir[1] / c.dist, c.dir[2] / c.dist);
-            if (trace_any<COUNT, FAR>(sc, org, dir, 0.0f, c.dist, st, cnt)) w = 0.0f;
+            if (trace_any<COUNT, FAR, STRI>(sc, org, dir, 0.0f, c.dist, st, cnt)) w = 0.0f;
         }
         if (w > 0.0f) {
             float val[3];
@@ -712,7 +759,7 @@ __device__ __forceinline__ float3 shadow_origin(const ShadeParams& sp, const flo
 // Lambert term of light j at hit point P with the two-sided normal N: the
 // contribution E cos is formed before the shadow query, so only it (not E
 // and the cosine) is live across the traversal; added to acc when unoccluded.
-template <bool COUNT, bool FAR = false>
+template <bool COUNT, bool FAR = false, bool STRI = false>
 __device__ __forceinline__ void light_term(const SceneView& sc, const ShadeParams& sp, int j,
                                            const float3& P, const float3& N, float3& acc,
                                            const Stack& st, TraceCount& cnt, uint32_t& nshadow)
@@ -725,7 +772,7 @@ __device__ __forceinline__ void light_term(const SceneView& sc, const ShadeParam
         const float3 c = make_float3(E.x * cosNL, E.y * cosNL, E.z * cosNL);
         const float3 org = shadow_origin(sp, P, N);
         ++nshadow;
-        if (!trace_any<COUNT, FAR>(sc, org, Ld, 0.0f, dist, st, cnt)) {
+        if (!trace_any<COUNT, FAR, STRI>(sc, org, Ld, 0.0f, dist, st, cnt)) {
             acc.x = acc.x + c.x;
             acc.y = acc.y + c.y;
             acc.z = acc.z + c.z;
@@ -754,14 +801,15 @@ __device__ __forceinline__ void hit_point_normal(const SceneView& sc, int prim, 
 // Radiance carried by one primary ray.  `nshadow` counts traced shadow rays.
 // IND compiles the multi-bounce term in (used only when sp.ind.coeffs is
 // set); direct-only kernels are built without it and keep its registers free.
-// FAR: shadow queries visit children far first (same radiance).
-template <bool COUNT, bool IND = false, bool FAR = false>
+// FAR: shadow queries visit children far first (same radiance).  STRI:
+// uniform triangle records through the scalar cache (fetch_wt).
+template <bool COUNT, bool IND = false, bool FAR = false, bool STRI = false>
 __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const float3& o,
                             const float3& d, float tmin, float tmax, const Stack& st,
                             TraceCount& cnt, uint32_t& nshadow, int& hit_prim)
 {
     float t = 0.0f, u = 0.0f, v = 0.0f;
-    const int prim = trace_closest<COUNT>(sc, o, d, tmin, tmax, t, u, v, st, cnt);
+    const int prim = trace_closest<COUNT, STRI>(sc, o, d, tmin, tmax, t, u, v, st, cnt);
     hit_prim = prim;
     const bool ref = sp.mode == VLB_SHADE_REFERENCE;
     if (prim < 0) {
@@ -797,7 +845,7 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
         bool in_shadow = true;
         if (sDotN != 0.0f) {
             ++nshadow;
-            in_shadow = trace_any<COUNT, FAR>(sc, org, Ln, 0.0f, Llen, st, cnt);
+            in_shadow = trace_any<COUNT, FAR, STRI>(sc, org, Ln, 0.0f, Llen, st, cnt);
         }
         float diffuse = 0.0f, spec = 0.0f;
         if (!in_shadow) {
@@ -836,8 +884,8 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
     // neither the hit point nor the normal is live across its traversal
     // (unless the indirect term needs them)
     const int nl = sp.nlights;
-    for (int j = 0; j + 1 < nl; ++j) light_term<COUNT, FAR>(sc, sp, j, P, N, acc, st, cnt, nshadow);
-    if (nl > 0) light_term<COUNT, FAR>(sc, sp, nl - 1, P, N, acc, st, cnt, nshadow);
+    for (int j = 0; j + 1 < nl; ++j) light_term<COUNT, FAR, STRI>(sc, sp, j, P, N, acc, st, cnt, nshadow);
+    if (nl > 0) light_term<COUNT, FAR, STRI>(sc, sp, nl - 1, P, N, acc, st, cnt, nshadow);
     // albedo read after the shadow queries rather than held across them (the
     // compiler barrier keeps it from reusing the loads of the normals, which
     // it would otherwise keep in scratch across the traversals)
@@ -847,7 +895,7 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
     const float kInvPi = 0.318309886183790671538f;
     if (!IND || !sp.ind.coeffs)
         return make_float3(alb.x * acc.x * kInvPi, alb.y * acc.y * kInvPi, alb.z * acc.z * kInvPi);
-    const float3 E = grid_indirect<COUNT, FAR>(sc, sp.ind, P, N, st, cnt, nshadow);
+    const float3 E = grid_indirect<COUNT, FAR, STRI>(sc, sp.ind, P, N, st, cnt, nshadow);
     return make_float3(alb.x * acc.x * kInvPi + alb.x * E.x, alb.y * acc.y * kInvPi + alb.y * E.y,
                        alb.z * acc.z * kInvPi + alb.z * E.z);
 }
