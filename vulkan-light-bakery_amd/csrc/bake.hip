@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_trace(BakeArgs a)
         if (probe < a.P && dd.w != 0.0f) {
             const float4 pp = a.probes[probe];
             int hp = -1;
-            L = shade_ray<COUNT, IND, FAR>(a.sc, a.sp, make_float3(pp.x, pp.y, pp.z),
+            L = shade_ray<COUNT, IND, FAR, kStriShadow>(a.sc, a.sp, make_float3(pp.x, pp.y, pp.z),
                                  make_float3(dd.x, dd.y, dd.z), a.tmin, a.tmax, st, cnt, nshadow, hp);
             if (RECORD) {
                 const size_t ri = (size_t)a.probe_perm[probe] * a.R + a.dir_perm[r];

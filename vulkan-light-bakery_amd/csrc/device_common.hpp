@@ -397,10 +397,13 @@ __device__ __forceinline__ BNode4 load_node_uniform(const QNode4* p)
 // scalar operands, sparing the vector memory path three 16-byte loads per
 // lane; else, and without SCALAR, each lane loads its own.  Each path runs
 // its own test, so the scalar words are never copied into vector registers.
-// The multi-job kernel takes SCALAR (a batch of 64 jobs over four scenes:
-// 197.0 vs 202.9 ms per trace stage); the single-bake kernel does not (on the
-// 11k-triangle room bake it ran 1.03-1.04 vs 0.99-1.00 ms: the extra scalar
-// registers spill there, into vector-register lanes read back in the loop).
+// The multi-job kernel takes SCALAR in both queries (a batch of 64 jobs over
+// four scenes: 196.4 vs 202.9 ms per trace stage; 200.0 with the shadow
+// queries alone).  The single-bake kernel takes it in the shadow queries
+// only (1.15M-triangle terrain, 16,384 probes x 16,384 rays: 38.25 vs 39.27
+// ms; the 11k-triangle room bake unchanged): in its closest-hit query the
+// extra scalar registers spill into vector-register lanes read back in the
+// loop, and the room bake ran 1.03-1.04 vs 0.99-1.00 ms.
 typedef int sgpr8_t __attribute__((ext_vector_type(8)));
 typedef int sgpr4_t __attribute__((ext_vector_type(4)));
 
@@ -802,14 +805,18 @@ __device__ __forceinline__ void hit_point_normal(const SceneView& sc, int prim, 
 // IND compiles the multi-bounce term in (used only when sp.ind.coeffs is
 // set); direct-only kernels are built without it and keep its registers free.
 // FAR: shadow queries visit children far first (same radiance).  STRI:
-// uniform triangle records through the scalar cache (fetch_wt).
-template <bool COUNT, bool IND = false, bool FAR = false, bool STRI = false>
+// where uniform triangle records come through the scalar cache (fetch_wt):
+// kStriClosest in the closest-hit query, kStriShadow in the shadow queries.
+enum : int { kStriClosest = 1, kStriShadow = 2 };
+
+template <bool COUNT, bool IND = false, bool FAR = false, int STRI = 0>
 __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const float3& o,
                             const float3& d, float tmin, float tmax, const Stack& st,
                             TraceCount& cnt, uint32_t& nshadow, int& hit_prim)
 {
     float t = 0.0f, u = 0.0f, v = 0.0f;
-    const int prim = trace_closest<COUNT, STRI>(sc, o, d, tmin, tmax, t, u, v, st, cnt);
+    constexpr bool kShadowScalar = (STRI & kStriShadow) != 0;
+    const int prim = trace_closest<COUNT, (STRI & kStriClosest) != 0>(sc, o, d, tmin, tmax, t, u, v, st, cnt);
     hit_prim = prim;
     const bool ref = sp.mode == VLB_SHADE_REFERENCE;
     if (prim < 0) {
@@ -845,7 +852,7 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
         bool in_shadow = true;
         if (sDotN != 0.0f) {
             ++nshadow;
-            in_shadow = trace_any<COUNT, FAR, STRI>(sc, org, Ln, 0.0f, Llen, st, cnt);
+            in_shadow = trace_any<COUNT, FAR, kShadowScalar>(sc, org, Ln, 0.0f, Llen, st, cnt);
         }
         float diffuse = 0.0f, spec = 0.0f;
         if (!in_shadow) {
@@ -884,8 +891,9 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
     // neither the hit point nor the normal is live across its traversal
     // (unless the indirect term needs them)
     const int nl = sp.nlights;
-    for (int j = 0; j + 1 < nl; ++j) light_term<COUNT, FAR, STRI>(sc, sp, j, P, N, acc, st, cnt, nshadow);
-    if (nl > 0) light_term<COUNT, FAR, STRI>(sc, sp, nl - 1, P, N, acc, st, cnt, nshadow);
+    for (int j = 0; j + 1 < nl; ++j)
+        light_term<COUNT, FAR, kShadowScalar>(sc, sp, j, P, N, acc, st, cnt, nshadow);
+    if (nl > 0) light_term<COUNT, FAR, kShadowScalar>(sc, sp, nl - 1, P, N, acc, st, cnt, nshadow);
     // albedo read after the shadow queries rather than held across them (the
     // compiler barrier keeps it from reusing the loads of the normals, which
     // it would otherwise keep in scratch across the traversals)
@@ -895,7 +903,7 @@ __device__ float3 shade_ray(const SceneView& sc, const ShadeParams& sp, const fl
     const float kInvPi = 0.318309886183790671538f;
     if (!IND || !sp.ind.coeffs)
         return make_float3(alb.x * acc.x * kInvPi, alb.y * acc.y * kInvPi, alb.z * acc.z * kInvPi);
-    const float3 E = grid_indirect<COUNT, FAR, STRI>(sc, sp.ind, P, N, st, cnt, nshadow);
+    const float3 E = grid_indirect<COUNT, FAR, kShadowScalar>(sc, sp.ind, P, N, st, cnt, nshadow);
     return make_float3(alb.x * acc.x * kInvPi + alb.x * E.x, alb.y * acc.y * kInvPi + alb.y * E.y,
                        alb.z * acc.z * kInvPi + alb.z * E.z);
 }

@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256, OCC > 0 ? OCC : 1) void k_fused_jobs(MultiArgs
         if (probe < J.P && dd.w != 0.0f) {
             const float4 pp = J.probes[probe];
             int hp = -1;
-            L = shade_ray<COUNT, false, FAR, true>(J.sc, J.sp, make_float3(pp.x, pp.y, pp.z), make_float3(dd.x, dd.y, dd.z),
+            L = shade_ray<COUNT, false, FAR, kStriClosest | kStriShadow>(J.sc, J.sp, make_float3(pp.x, pp.y, pp.z), make_float3(dd.x, dd.y, dd.z),
                                  J.tmin, J.tmax, st, cnt, nshadow, hp);
         }
         float* out = a.rad + (size_t)tile * 1536 + l * 8 + e;
