@@ -261,10 +261,12 @@ def serve(ctx, vlb, W, host, sids, args, oracle_of=None):
                      step_rays=args.step_rays, policy=SCHED[args.policy], trace_variant=args.variant,
                      age_ms=args.age_ms)
     # warm-up: one small request per scene (direction tables, graph capture)
+    warm = []
     for si in range(len(host)):
         w = (si, reqs[0][1][:256], sun0,
              [(vlb.LIGHT_DIRECTIONAL, (-sun0).tolist(), (3.0, 3.0, 3.0))])
-        srv.wait(srv.submit(sids[si], W.c5_params(w, PREC["bf16"])))
+        warm.append((si, W.c5_params(w, PREC["bf16"])))
+        srv.wait(srv.submit(sids[si], warm[-1][1]))
     ps16 = [W.c5_params(r, PREC["bf16"]) for r in reqs]
     ids = [None] * n
 
@@ -321,7 +323,43 @@ def serve(ctx, vlb, W, host, sids, args, oracle_of=None):
                                       "p50_ms": round(float(np.percentile(plat, 50)), 2),
                                       "p99_ms": round(float(np.percentile(plat, 99)), 2),
                                       "span_s": round(plain_span, 3)}}
-    return res, reqs, ps16, ps32, outs
+    return res, reqs, ps16, ps32, outs, warm
+
+
+def served_roofline(ctx, sids, reqs, ps16, warm, server):
+    """Roofline of the served form's trace work.  The server's steps run
+    the multi-job trace kernel on mixed requests; a step's device time is
+    event-timed by the server (busy_ms, which also holds the step's small
+    projection and merge, so the rates below are lower bounds for the trace
+    kernel).  Algorithmic bytes: node visits and triangle tests of every
+    ray the server traced (the requests and the warm-up requests), counted
+    by one batch counting pass of the same variant -- a ray's counts do not
+    depend on how rays are grouped into launches.  `traffic` is the HBM
+    bytes per step from the committed rocprofv3 PMC summary of this
+    workload."""
+    v = server["trace_variant"]
+    jobs = [(sids[si], p) for si, p in warm] + [(sids[reqs[i][0]], ps16[i]) for i in range(len(reqs))]
+    b = ctx.batch(jobs, trace_variant=v)
+    b.count()
+    cnt = b.info()
+    b.destroy()
+    steps = server["steps"]
+    step_s = server["busy_ms"] / 1e3 / steps
+    alg = (node_bytes(v) * cnt["node_visits"] + TRI_BYTES * cnt["tri_tests"]) / steps
+    profile = load_traffic("served_c5_traffic.json")
+    prof = (profile or {}).get("variants", {}).get(str(v))
+    traffic = prof["hbm_bytes_per_launch"] if prof else None
+    achieved = traffic / step_s / 1e9 if traffic else None
+    return {"bound": "hbm", "achieved": None if achieved is None else round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic, "kernel": "k_fused_jobs (server steps)", "variant": v,
+            "launch_ms": round(step_s * 1e3, 4), "launch_timing": "server step device time (HIP events)",
+            "launches": steps, "alg_bytes_per_launch": round(alg, 1),
+            "alg_rate_gbs": round(alg / step_s / 1e9, 1),
+            "node_visits": cnt["node_visits"], "tri_tests": cnt["tri_tests"],
+            "issue": prof.get("issue") if prof else None,
+            "profile": (profile or {}).get("file") if prof else None}
 
 
 # ----------------------------------------------------------- batched form
@@ -489,7 +527,8 @@ def bench_c5(args):
     host = W.scenes(W.C5_SCENES)
     sids = [ctx.upload_scene_dict(s) for s in host]
     infos = [ctx.scene_info(s) for s in sids]
-    res, reqs, ps16, ps32, outs = serve(ctx, vlb, W, host, sids, args)
+    res, reqs, ps16, ps32, outs, warm = serve(ctx, vlb, W, host, sids, args)
+    roof = served_roofline(ctx, sids, reqs, ps16, warm, res["server"])
     orc = oracle_leg(args)
     err = cpu = hits = None
     if orc is not None:
@@ -512,7 +551,7 @@ def bench_c5(args):
             q = vlb.Params(**{**ps32[i].__dict__, "_keep": []})
             q.probes = pr[np.sort(rng.choice(pr.shape[0], size=2, replace=False))].copy()
             picks.append((si, q))
-        # the variant the server ran (its default is 11)
+        # the variant the server ran (its default is 35)
         hits = _hit_parity(ctx, vlb, host, sids, oracle_of, picks, res["server"]["trace_variant"])
         spp_scene, t_cpu = {}, 0.0
         for i in range(n):
@@ -538,7 +577,7 @@ def bench_c5(args):
                                    "bvh_depth": infos[i]["depth"]} for i in range(len(host))]},
             "latency_ms": lat, "server": res["server"],
             "plain_fp32_per_request": res["plain_fp32_per_request"],
-            "sh_rel_err_vs_fp64": err, "hit_parity": hits, "cpu_baseline": cpu}
+            "sh_rel_err_vs_fp64": err, "hit_parity": hits, "roofline": roof, "cpu_baseline": cpu}
 
 
 # -------------------------------------------------------- plain single bake
